@@ -1,0 +1,488 @@
+// Host driver and C ABI of libdlu: the delayed-pivoting LU on one MI355X.
+//
+// Reference interface replaced: delayed_lu_cuda/delayed_dgetrf_cuda.hpp:7-63 and
+// delayed_lu_cuda/delayed_dgetrf_cuda.cu:370-517 (argument checks, status mapping, device/host
+// entry points, aliases, status strings).  Algorithm control flow restated from
+// delayed_lu/delayed_dgetrf.cpp:129-251 (panel loop, full-pivot fallback, panel restart,
+// deferral) with decisions taken on the device:
+//   * the panel kernel chooses every column pivot and stops by itself at a zero column;
+//   * the full-pivot search, the interchanges it implies and the deferral flag are kernels too;
+//   * the host reads a small control block once per panel (not once per column) to learn where
+//     the panel stopped, and uses it only to size the next launches.
+// Look-ahead: after panel j, the columns of panel j+1 are updated and factored on the caller's
+// stream while the update of the far columns runs on a second stream of the same device.
+#include "../../include/dlu.h"
+#include "dlu_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+namespace dlu {
+namespace {
+
+// delayed_lu_cuda/delayed_dgetrf_cuda.cu:370-383: lda*n*8 must not overflow size_t.
+bool matrix_bytes(int lda, int n, size_t &bytes) {
+  const size_t cols = static_cast<size_t>(n), lead = static_cast<size_t>(lda);
+  if (cols != 0 && lead > std::numeric_limits<size_t>::max() / cols) return false;
+  const size_t elems = lead * cols;
+  if (elems > std::numeric_limits<size_t>::max() / sizeof(double)) return false;
+  bytes = elems * sizeof(double);
+  return true;
+}
+
+// delayed_lu_cuda/delayed_dgetrf_cuda.cu:385-397 and delayed_lu/delayed_dgetrf.cpp:132-142.
+bool valid_arguments(int m, int n, const double *a, int lda, const int *ipiv, const int *jpiv) {
+  if (m < 0 || n < 0 || lda < std::max(1, m)) return false;
+  const int steps = std::min(m, n);
+  if ((m > 0 && n > 0 && a == nullptr) || (steps > 0 && (ipiv == nullptr || jpiv == nullptr)))
+    return false;
+  size_t ignored = 0;
+  return matrix_bytes(lda, n, ignored);
+}
+
+// Per-device state reused across calls (one factorization at a time per device).
+struct Workspace {
+  int device = -1;
+  int steps_cap = 0;
+  int cus = 0;
+  Control *d_ctl = nullptr;
+  Control *h_ctl = nullptr;
+  int *d_ipiv = nullptr;
+  int *d_jpiv = nullptr;
+  double *d_margins = nullptr;
+  uint64_t *d_slots = nullptr;
+  uint64_t *d_rowk = nullptr;
+  int *d_lists = nullptr;
+  double *d_pv = nullptr;
+  unsigned long long *d_po = nullptr;
+  unsigned seq = 0;
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_ctl = nullptr, ev_aux = nullptr, ev_start = nullptr;
+  bool ready = false;
+};
+
+constexpr int kMaxDevices = 64;
+constexpr int kSearchPartials = 2048;
+constexpr int kListStride = 4 * 256 + 4;  // swap_list_ints(256), enough for every nb
+
+std::mutex g_lock;
+Workspace g_ws[kMaxDevices];
+
+hipError_t prepare(Workspace &w, int device, int steps) {
+  if (!w.ready) {
+    w.device = device;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return e;
+    w.cus = prop.multiProcessorCount;
+    const size_t slot_bytes =
+        2ull * kMaxPanelGroups * static_cast<size_t>(slot_words(256)) * sizeof(uint64_t);
+    if ((e = hipMalloc(&w.d_ctl, sizeof(Control))) != hipSuccess) return e;
+    if ((e = hipHostMalloc(&w.h_ctl, sizeof(Control))) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_slots, slot_bytes)) != hipSuccess) return e;
+    if ((e = hipMemset(w.d_slots, 0, slot_bytes)) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_rowk, 2 * 2 * 256 * sizeof(uint64_t))) != hipSuccess) return e;
+    if ((e = hipMemset(w.d_rowk, 0, 2 * 2 * 256 * sizeof(uint64_t))) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_lists, 2 * kListStride * sizeof(int))) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_pv, kSearchPartials * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_po, kSearchPartials * sizeof(unsigned long long))) != hipSuccess) return e;
+    if ((e = hipStreamCreateWithFlags(&w.aux, hipStreamNonBlocking)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&w.ev_ctl, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&w.ev_aux, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&w.ev_start, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
+    w.ready = true;
+  }
+  if (steps > w.steps_cap) {
+    hipFree(w.d_ipiv);
+    hipFree(w.d_jpiv);
+    hipFree(w.d_margins);
+    w.d_ipiv = w.d_jpiv = nullptr;
+    w.d_margins = nullptr;
+    w.steps_cap = 0;
+    hipError_t e;
+    if ((e = hipMalloc(&w.d_ipiv, steps * sizeof(int))) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_jpiv, steps * sizeof(int))) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_margins, steps * sizeof(double))) != hipSuccess) return e;
+    w.steps_cap = steps;
+  }
+  return hipSuccess;
+}
+
+inline double *at(double *a, int lda, int i, int j) {
+  return a + static_cast<size_t>(i) + static_cast<size_t>(j) * static_cast<size_t>(lda);
+}
+
+struct Run {
+  Workspace &w;
+  int m, n, lda, steps, nb;
+  double *a;
+  hipStream_t s;
+  long long spin;
+  bool profile = false;
+  int panels = 0;
+
+  // Opt-in launch timing: an event pair around each launch, on the launch's own stream.
+  struct Timed {
+    hipEvent_t begin, end;
+    double flops;
+  };
+  std::vector<Timed> t_update, t_panel, t_solve;
+  hipError_t mark(std::vector<Timed> &v, hipStream_t st, double flops) {
+    Timed t{nullptr, nullptr, flops};
+    hipError_t e = hipEventCreate(&t.begin);
+    if (e == hipSuccess) e = hipEventCreate(&t.end);
+    if (e == hipSuccess) e = hipEventRecord(t.begin, st);
+    v.push_back(t);
+    return e;
+  }
+  hipError_t close(std::vector<Timed> &v, hipStream_t st) { return hipEventRecord(v.back().end, st); }
+  static double total(const std::vector<Timed> &v, double *flops) {
+    double sec = 0.0, fl = 0.0;
+    for (const Timed &t : v) {
+      float ms = 0.0f;
+      if (t.begin && t.end && hipEventElapsedTime(&ms, t.begin, t.end) == hipSuccess) sec += ms * 1e-3;
+      fl += t.flops;
+    }
+    if (flops) *flops = fl;
+    return sec;
+  }
+  void release() {
+    for (auto *v : {&t_update, &t_panel, &t_solve})
+      for (Timed &t : *v) {
+        if (t.begin) hipEventDestroy(t.begin);
+        if (t.end) hipEventDestroy(t.end);
+      }
+  }
+
+  // Rows per lane of the panel kernel: the panel's rows must fit in the co-resident workgroups.
+  bool panel_shape(int pb, int &groups, int &rpl) const {
+    const int rows = m - pb;
+    const int max_groups = std::min(kMaxPanelGroups, w.cus > 0 ? w.cus : kMaxPanelGroups);
+    const int rpl_max = nb == 256 ? 2 : 4;
+    for (rpl = 1; rpl <= rpl_max; rpl *= 2) {
+      groups = (rows + 64 * rpl - 1) / (64 * rpl);
+      if (groups <= max_groups) return true;
+    }
+    return false;
+  }
+
+  hipError_t panel(int pb, int pe, int positioned, int parity) {
+    int groups = 0, rpl = 0;
+    if (!panel_shape(pb, groups, rpl)) return hipErrorInvalidValue;
+    PanelParams p{};
+    p.a = a;
+    p.lda = lda;
+    p.m = m;
+    p.pb = pb;
+    p.pe = pe;
+    p.positioned = positioned;
+    p.groups = groups;
+    p.seq_base = w.seq;
+    w.seq += static_cast<unsigned>(nb) + 1u;
+    p.ipiv = w.d_ipiv;
+    p.margins = w.d_margins;
+    p.ctl = w.d_ctl;
+    p.slots = w.d_slots;
+    p.rowk = w.d_rowk;
+    p.swap_list = w.d_lists + parity * kListStride;
+    p.spin_limit = spin;
+    hipError_t e = profile ? mark(t_panel, s, 0.0) : hipSuccess;
+    if (e == hipSuccess) e = launch_panel(p, nb, rpl, s);
+    if (e == hipSuccess && profile) e = close(t_panel, s);
+    if (e != hipSuccess) return e;
+    ++panels;
+    if ((e = hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost, s)) !=
+        hipSuccess)
+      return e;
+    return hipEventRecord(w.ev_ctl, s);
+  }
+
+  // Interchange + U block row + trailing update of columns [c0, c1) for the panel rows
+  // [pb, pb+w), updating rows [pb+w, m).
+  hipError_t update(int pb, int wdt, int c0, int c1, const int *list, hipStream_t st) {
+    if (c1 <= c0 || wdt <= 0) return hipSuccess;
+    hipError_t e = profile ? mark(t_solve, st, 0.0) : hipSuccess;
+    if (e == hipSuccess) e = launch_swap_trsm(a, lda, pb, wdt, c0, c1, list, nb, st);
+    if (e == hipSuccess && profile) e = close(t_solve, st);
+    if (e != hipSuccess) return e;
+    const int r0 = pb + wdt;
+    if (m - r0 <= 0) return hipSuccess;
+    const double flops = 2.0 * (m - r0) * static_cast<double>(c1 - c0) * wdt;
+    if (profile && (e = mark(t_update, st, flops)) != hipSuccess) return e;
+    e = launch_gemm_sub(at(a, lda, r0, c0), at(a, lda, r0, pb), at(a, lda, pb, c0), lda, m - r0,
+                        c1 - c0, wdt, st);
+    if (e == hipSuccess && profile) e = close(t_update, st);
+    return e;
+  }
+
+  hipError_t factor(int *info, bool lookahead) {
+    hipError_t e;
+    int pb = 0, parity = 0;
+    bool aux_pending = false;
+    int pe = std::min(pb + nb, steps);
+    if ((e = panel(pb, pe, 0, parity)) != hipSuccess) return e;
+    for (;;) {
+      if ((e = hipEventSynchronize(w.ev_ctl)) != hipSuccess) return e;
+      const Control c = *w.h_ctl;
+      if (c.error != 0) return hipErrorLaunchFailure;
+      if (c.deferred) {
+        *info = c.stop;
+        break;
+      }
+      const int stop = c.stop;
+      const int wdt = stop - pb;
+      const int *list = w.d_lists + parity * kListStride;
+      if (!c.zero_column) {
+        const int pe2 = std::min(pe + nb, steps);
+        if (pe < steps && lookahead) {
+          // Next panel's columns on the caller's stream, far columns on the auxiliary stream.
+          if (aux_pending && (e = hipStreamWaitEvent(s, w.ev_aux, 0)) != hipSuccess) return e;
+          if ((e = update(pb, wdt, pe, pe2, list, s)) != hipSuccess) return e;
+          if ((e = hipStreamWaitEvent(w.aux, w.ev_ctl, 0)) != hipSuccess) return e;
+          if ((e = update(pb, wdt, pe2, n, list, w.aux)) != hipSuccess) return e;
+          if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, w.aux)) != hipSuccess) return e;
+          if ((e = hipEventRecord(w.ev_aux, w.aux)) != hipSuccess) return e;
+          aux_pending = true;
+          pb = pe;
+          pe = pe2;
+          parity ^= 1;
+          if ((e = panel(pb, pe, 0, parity)) != hipSuccess) return e;
+          continue;
+        }
+        if (aux_pending) {
+          if ((e = hipStreamWaitEvent(s, w.ev_aux, 0)) != hipSuccess) return e;
+          aux_pending = false;
+        }
+        if ((e = update(pb, wdt, pe, n, list, s)) != hipSuccess) return e;
+        if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, s)) != hipSuccess) return e;
+        if (pe >= steps) {
+          *info = steps;
+          break;
+        }
+        pb = pe;
+        pe = pe2;
+        parity ^= 1;
+        if ((e = panel(pb, pe, 0, parity)) != hipSuccess) return e;
+        continue;
+      }
+      // Zero column at `stop`: bring the far columns up to date with the accepted prefix
+      // (delayed_dgetrf.cpp:183-186), search the whole active block, interchange, restart.
+      if (aux_pending) {
+        if ((e = hipStreamWaitEvent(s, w.ev_aux, 0)) != hipSuccess) return e;
+        aux_pending = false;
+      }
+      if (wdt > 0) {
+        if ((e = update(pb, wdt, pe, n, list, s)) != hipSuccess) return e;
+        if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, s)) != hipSuccess) return e;
+      }
+      if ((e = launch_active_search(a, lda, m, n, stop, w.d_pv, w.d_po, w.d_ctl, s)) != hipSuccess)
+        return e;
+      if ((e = launch_fallback_apply(a, lda, m, n, stop, w.d_ipiv, w.d_jpiv, w.d_ctl, s)) !=
+          hipSuccess)
+        return e;
+      pb = stop;
+      pe = std::min(pb + nb, steps);
+      parity ^= 1;
+      if ((e = panel(pb, pe, 1, parity)) != hipSuccess) return e;
+    }
+    if (aux_pending && (e = hipStreamWaitEvent(s, w.ev_aux, 0)) != hipSuccess) return e;
+    return hipSuccess;
+  }
+};
+
+dlu_status to_status(hipError_t e) {
+  if (e == hipSuccess) return DLU_SUCCESS;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return DLU_ALLOCATION_FAILED;
+  if (e == hipErrorLaunchFailure) return DLU_KERNEL_FAILURE;
+  return DLU_RUNTIME_FAILURE;
+}
+
+dlu_status getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv, int *info,
+                        hipStream_t stream, const dlu_options *opt, dlu_report *rep,
+                        double *margins) {
+  if (info == nullptr) return DLU_INVALID_ARGUMENT;
+  *info = -1;
+  if (!valid_arguments(m, n, d_a, lda, ipiv, jpiv)) return DLU_INVALID_ARGUMENT;
+  int nb = (opt && opt->nb) ? opt->nb : 256;
+  if (nb != 64 && nb != 128 && nb != 256) return DLU_INVALID_ARGUMENT;
+  const bool lookahead = !(opt && opt->lookahead < 0);
+  const long long spin =
+      (opt && opt->spin_limit_ticks > 0) ? opt->spin_limit_ticks : 200000000ll;
+  const int steps = std::min(m, n);
+  *info = steps;
+  for (int k = 0; k < steps; ++k) ipiv[k] = jpiv[k] = k;
+  if (rep) {
+    std::memset(rep, 0, sizeof *rep);
+    rep->min_margin = 2.0;
+  }
+  if (margins)
+    for (int k = 0; k < steps; ++k) margins[k] = -1.0;
+  if (steps == 0) return DLU_SUCCESS;
+  // Panels taller than 256 workgroups x 128 rows need the narrower panel kernels.
+  if (nb == 256 && m > 32768) nb = 128;
+
+  int device = 0;
+  hipError_t e = hipGetDevice(&device);
+  if (e != hipSuccess || device < 0 || device >= kMaxDevices) {
+    *info = -1;
+    return DLU_RUNTIME_FAILURE;
+  }
+  std::lock_guard<std::mutex> guard(g_lock);
+  Workspace &w = g_ws[device];
+  if ((e = prepare(w, device, steps)) != hipSuccess) {
+    *info = -1;
+    return to_status(e);
+  }
+  Run run{w, m, n, lda, steps, nb, d_a, stream, spin};
+  run.profile = opt && opt->profile;
+  {
+    int probe_groups = 0, probe_rpl = 0;
+    if (!run.panel_shape(0, probe_groups, probe_rpl)) {
+      *info = -1;
+      return DLU_INVALID_ARGUMENT;  // more rows than the panel kernel can hold
+    }
+  }
+  Control init{};
+  init.min_margin = 2.0;
+  *w.h_ctl = init;
+  if ((e = hipMemcpyAsync(w.d_ctl, w.h_ctl, sizeof(Control), hipMemcpyHostToDevice, stream)) ==
+          hipSuccess &&
+      (e = launch_init_steps(w.d_ipiv, w.d_jpiv, w.d_margins, steps, stream)) == hipSuccess) {
+    int out_info = steps;
+    e = run.factor(&out_info, lookahead);
+    if (e == hipSuccess) *info = out_info;
+  }
+  // Whatever happened, leave no work of this call queued on the auxiliary stream.
+  hipError_t e2 = hipStreamSynchronize(w.aux);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ipiv, w.d_ipiv, steps * sizeof(int), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(jpiv, w.d_jpiv, steps * sizeof(int), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess && margins)
+    e = hipMemcpyAsync(margins, w.d_margins, steps * sizeof(double), hipMemcpyDeviceToHost, stream);
+  e2 = hipStreamSynchronize(stream);
+  if (e == hipSuccess) e = e2;
+  if (e != hipSuccess) {
+    *info = -1;
+    run.release();
+    return to_status(e);
+  }
+  if (rep) {
+    const Control c = *w.h_ctl;
+    rep->panels = run.panels;
+    rep->fallbacks = c.fallbacks;
+    rep->near_ties = c.near_ties;
+    rep->exact_ties = c.exact_ties;
+    rep->min_margin = c.min_margin;
+    if (run.profile) {
+      rep->update_launches = static_cast<int>(run.t_update.size());
+      rep->update_seconds = Run::total(run.t_update, &rep->update_flops);
+      rep->panel_launches = static_cast<int>(run.t_panel.size());
+      rep->panel_seconds = Run::total(run.t_panel, nullptr);
+      rep->solve_launches = static_cast<int>(run.t_solve.size());
+      rep->solve_seconds = Run::total(run.t_solve, nullptr);
+    }
+  }
+  run.release();
+  return DLU_SUCCESS;
+}
+
+dlu_status getrf_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv, int *info,
+                      hipStream_t stream) {
+  if (info == nullptr) return DLU_INVALID_ARGUMENT;
+  *info = -1;
+  if (!valid_arguments(m, n, a, lda, ipiv, jpiv)) return DLU_INVALID_ARGUMENT;
+  if (std::min(m, n) == 0) {
+    *info = 0;  // delayed_lu_cuda/delayed_dgetrf_cuda.cu:456-460
+    return DLU_SUCCESS;
+  }
+  size_t bytes = 0;
+  if (!matrix_bytes(lda, n, bytes)) return DLU_INVALID_ARGUMENT;
+  double *d_a = nullptr;
+  if (hipMalloc(&d_a, bytes) != hipSuccess) return DLU_ALLOCATION_FAILED;
+  std::unique_ptr<double, hipError_t (*)(void *)> hold(d_a, hipFree);
+  if (hipMemcpyAsync(d_a, a, bytes, hipMemcpyHostToDevice, stream) != hipSuccess)
+    return DLU_RUNTIME_FAILURE;
+  const dlu_status st = getrf_device(m, n, d_a, lda, ipiv, jpiv, info, stream, nullptr, nullptr,
+                                     nullptr);
+  if (st != DLU_SUCCESS) return st;
+  if (hipMemcpyAsync(a, d_a, bytes, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess) {
+    *info = -1;
+    return DLU_RUNTIME_FAILURE;
+  }
+  return DLU_SUCCESS;
+}
+
+}  // namespace
+}  // namespace dlu
+
+extern "C" {
+
+dlu_status dlu_getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv, int *info,
+                            void *stream) {
+  return dlu::getrf_device(m, n, d_a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream),
+                           nullptr, nullptr, nullptr);
+}
+
+dlu_status dlu_getrf_device_ex(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
+                               int *info, void *stream, const dlu_options *options,
+                               dlu_report *report, double *margins) {
+  return dlu::getrf_device(m, n, d_a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream),
+                           options, report, margins);
+}
+
+dlu_status dlu_getrf_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv, int *info,
+                          void *stream) {
+  return dlu::getrf_host(m, n, a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream));
+}
+
+dlu_status dlu_getrf_native_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
+                                   int *info, void *stream) {
+  return dlu_getrf_device(m, n, d_a, lda, ipiv, jpiv, info, stream);
+}
+
+dlu_status dlu_getrf_native_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                                 int *info, void *stream) {
+  return dlu_getrf_host(m, n, a, lda, ipiv, jpiv, info, stream);
+}
+
+dlu_status dlu_getrf_hybrid_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
+                                   int *info, void *stream) {
+  return dlu_getrf_device(m, n, d_a, lda, ipiv, jpiv, info, stream);
+}
+
+dlu_status dlu_getrf_hybrid_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                                 int *info, void *stream) {
+  return dlu_getrf_host(m, n, a, lda, ipiv, jpiv, info, stream);
+}
+
+const char *dlu_status_string(dlu_status status) {
+  switch (status) {
+    case DLU_SUCCESS: return "success";
+    case DLU_INVALID_ARGUMENT: return "invalid argument";
+    case DLU_ALLOCATION_FAILED: return "HIP allocation failed";
+    case DLU_RUNTIME_FAILURE: return "HIP runtime failure";
+    case DLU_KERNEL_FAILURE: return "kernel failure";
+  }
+  return "unknown status";
+}
+
+dlu_status dlu_gemm_sub_device(int M, int N, int K, double *C, const double *A, const double *B,
+                               int ld, void *stream) {
+  if (M < 0 || N < 0 || K < 0 || ld < std::max(1, M) || (M > 0 && N > 0 && C == nullptr) ||
+      (M > 0 && K > 0 && A == nullptr) || (K > 0 && N > 0 && B == nullptr) || ld < K)
+    return DLU_INVALID_ARGUMENT;
+  return dlu::to_status(
+      dlu::launch_gemm_sub(C, A, B, ld, M, N, K, static_cast<hipStream_t>(stream)));
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// Internal declarations shared by the HIP translation units of libdlu.
+//
+// Device-side state of one factorization lives in a Workspace (see dlu_getrf.cpp).  The control
+// block below is written by kernels and read by later kernels (predication) and, once per panel,
+// by the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace dlu {
+
+// Control block written on the device.  One per workspace.
+struct Control {
+  int stop;          // panel kernel: steps [pb, stop) of the last panel are eliminated
+  int zero_column;   // panel kernel: 1 when it stopped because column `stop` is zero
+  int deferred;      // fallback: 1 when the whole active block at `stop` was zero (info = stop)
+  int error;         // nonzero: a bounded wait expired (2) or an internal check failed (1)
+  int fallbacks;     // number of full-pivot fallbacks taken so far
+  int near_ties;     // column steps whose relative top-2 gap was below 1e-10
+  int exact_ties;    // column steps whose two largest magnitudes were equal
+  double min_margin; // smallest relative top-2 gap seen at a column step
+  // full-pivot search result (fallback)
+  double best_value;
+  int best_row;
+  int best_col;
+  unsigned search_ticket;
+  unsigned abort_flag;  // set by a panel workgroup whose wait expired; the others leave too
+};
+
+// Exchange buffer geometry of the panel kernel (in 64-bit words).  Each word carries a 32-bit
+// payload and the 32-bit step sequence number that wrote it, so a reader can tell a fresh word
+// from a stale one without any cache-maintenance fence.
+constexpr int kSlotHeaderWords = 8;
+inline constexpr int slot_words(int nb) { return kSlotHeaderWords + 2 * nb; }
+constexpr int kMaxPanelGroups = 256;
+
+// A panel's net row movement: [0, 2nb) destination rows, [2nb, 4nb) source rows, [4nb] count.
+// Two such lists alternate between consecutive panels so the far update of panel j can still
+// read its list while panel j+1 writes the next one.
+inline constexpr int swap_list_ints(int nb) { return 4 * nb + 4; }
+
+struct PanelParams {
+  double *a;
+  int lda;
+  int m;
+  int pb;         // first column (and row) of the panel
+  int pe;         // one past the last column of the panel
+  int positioned; // the pivot of step pb is already in place (after a full-pivot fallback)
+  int groups;     // number of workgroups (row blocks) taking part
+  unsigned seq_base;
+  int *ipiv;           // device ipiv (0-based swap history)
+  double *margins;     // device per-step relative top-2 gap (may be null)
+  Control *ctl;
+  uint64_t *slots;     // [2][groups][slot_words(nb)]
+  uint64_t *rowk;      // [2][2*nb]
+  int *swap_list;      // swap_list_ints(nb) ints for this panel (parity selected by the host)
+  long long spin_limit;  // bounded wait, in s_memrealtime ticks (100 MHz)
+};
+
+// Launchers (return hipSuccess or the launch error).
+hipError_t launch_panel(const PanelParams &p, int nb, int rows_per_lane, hipStream_t s);
+
+// Right-side interchange + triangular solve: rows [pb, pb+w) of columns [c0, c1) become
+// U12 = L11^{-1} * (rows after interchange); rows displaced below pb+w are written directly.
+hipError_t launch_swap_trsm(double *a, int lda, int pb, int w, int c0, int c1,
+                            const int *swap_list, int nb, hipStream_t s);
+
+// C(M x N) -= A(M x K) * B(K x N), column-major, all with leading dimension lda.
+hipError_t launch_gemm_sub(double *c, const double *a, const double *b, int lda, int M, int N,
+                           int K, hipStream_t s);
+
+// Row interchanges of the last panel applied to columns [c0, c1) (left of the panel).
+hipError_t launch_row_interchange(double *a, int lda, int c0, int c1, const int *swap_list,
+                                  int nb, hipStream_t s);
+
+// Identity swap histories and "unreached" margins for a new factorization.
+hipError_t launch_init_steps(int *ipiv, int *jpiv, double *margins, int steps, hipStream_t s);
+
+// Full-pivot search over the active block A(k:m-1, k:n-1); result in ctl->best_*.
+hipError_t launch_active_search(const double *a, int lda, int m, int n, int k, double *partial_v,
+                                unsigned long long *partial_o, Control *ctl, hipStream_t s);
+
+// Apply the full-pivot decision at step k (row swap, then column swap) or mark the deferral.
+hipError_t launch_fallback_apply(double *a, int lda, int m, int n, int k, int *ipiv, int *jpiv,
+                                 Control *ctl, hipStream_t s);
+
+}  // namespace dlu

@@ -1,0 +1,164 @@
+"""ctypes binding of libdlu.so (the C ABI declared in include/dlu.h).
+
+Mirrors the reference's GPU interface (delayed_lu_cuda/delayed_dgetrf_cuda.hpp:7-63): the same
+argument meaning, status ordinals and ``info`` rules.  Device buffers come from PyTorch (plumbing
+only); the factorization itself always runs in libdlu's HIP kernels.  There is no CPU fallback:
+if the library is missing or a GPU call fails, these functions raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libdlu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "dlu.h")
+
+
+class Status(enum.IntEnum):
+    SUCCESS = 0
+    INVALID_ARGUMENT = 1
+    ALLOCATION_FAILED = 2
+    RUNTIME_FAILURE = 3
+    KERNEL_FAILURE = 4
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("nb", ctypes.c_int), ("lookahead", ctypes.c_int),
+                ("spin_limit_ticks", ctypes.c_longlong), ("profile", ctypes.c_int)]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [("panels", ctypes.c_int), ("fallbacks", ctypes.c_int),
+                ("near_ties", ctypes.c_int), ("exact_ties", ctypes.c_int),
+                ("min_margin", ctypes.c_double),
+                ("update_launches", ctypes.c_int), ("update_seconds", ctypes.c_double),
+                ("update_flops", ctypes.c_double), ("panel_launches", ctypes.c_int),
+                ("panel_seconds", ctypes.c_double), ("solve_launches", ctypes.c_int),
+                ("solve_seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    """Load libdlu.so; raise if it has not been built (no silent fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libdlu.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip,
+           ctypes.c_void_p]
+    for name in ("dlu_getrf_device", "dlu_getrf_host", "dlu_getrf_native_device",
+                 "dlu_getrf_native_host", "dlu_getrf_hybrid_device", "dlu_getrf_hybrid_host"):
+        fn = getattr(L, name)
+        fn.argtypes = sig
+        fn.restype = ctypes.c_int
+    L.dlu_getrf_device_ex.argtypes = sig + [ctypes.POINTER(Options), ctypes.POINTER(Report), _dp]
+    L.dlu_getrf_device_ex.restype = ctypes.c_int
+    L.dlu_status_string.argtypes = [ctypes.c_int]
+    L.dlu_status_string.restype = ctypes.c_char_p
+    L.dlu_gemm_sub_device.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 3 + [
+        ctypes.c_int, ctypes.c_void_p]
+    L.dlu_gemm_sub_device.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def status_string(status: int) -> str:
+    return lib().dlu_status_string(int(status)).decode()
+
+
+def declared_symbols() -> list:
+    """Function names declared in include/dlu.h (for the export check)."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(dlu_\w+)\s*\(", text, re.M)))
+
+
+def _stream_ptr(stream) -> Optional[int]:
+    if stream is None:
+        return None
+    return int(getattr(stream, "cuda_stream", stream))
+
+
+def getrf_device(m: int, n: int, d_a: int, lda: int, stream=None, options: Optional[dict] = None,
+                 margins: bool = False, variant: str = "device"):
+    """Factor the device matrix at address ``d_a`` in place.
+
+    Returns ``(status, ipiv, jpiv, info, report, margins)``; ``report`` and ``margins`` are None
+    unless the extended entry point is used (``options`` given or ``margins=True``).
+    """
+    L = lib()
+    p = max(min(m, n), 0)
+    ipiv = np.zeros(max(p, 1), dtype=np.int32)
+    jpiv = np.zeros(max(p, 1), dtype=np.int32)
+    info = ctypes.c_int(-7)
+    ip = ipiv.ctypes.data_as(_ip)
+    jp = jpiv.ctypes.data_as(_ip)
+    s = _stream_ptr(stream)
+    if options is not None or margins:
+        opt = Options(**(options or {}))
+        rep = Report()
+        marg = np.zeros(max(p, 1), dtype=np.float64) if margins else None
+        st = L.dlu_getrf_device_ex(m, n, d_a, lda, ip, jp, ctypes.byref(info), s,
+                                   ctypes.byref(opt), ctypes.byref(rep),
+                                   marg.ctypes.data_as(_dp) if margins else None)
+        return (Status(st), ipiv[:p], jpiv[:p], info.value, rep.as_dict(),
+                marg[:p] if margins else None)
+    fn = {"device": L.dlu_getrf_device, "native": L.dlu_getrf_native_device,
+          "hybrid": L.dlu_getrf_hybrid_device}[variant]
+    st = fn(m, n, d_a, lda, ip, jp, ctypes.byref(info), s)
+    return Status(st), ipiv[:p], jpiv[:p], info.value, None, None
+
+
+def getrf_host(a: np.ndarray, variant: str = "host", stream=None):
+    """Host-memory interface on a Fortran-ordered copy of ``a``: (status, lu, ipiv, jpiv, info)."""
+    L = lib()
+    f = np.array(a, dtype=np.float64, order="F", copy=True)
+    m, n = f.shape
+    p = min(m, n)
+    ipiv = np.zeros(max(p, 1), dtype=np.int32)
+    jpiv = np.zeros(max(p, 1), dtype=np.int32)
+    info = ctypes.c_int(-7)
+    fn = {"host": L.dlu_getrf_host, "native": L.dlu_getrf_native_host,
+          "hybrid": L.dlu_getrf_hybrid_host}[variant]
+    st = fn(m, n, f.ctypes.data, max(1, m), ipiv.ctypes.data_as(_ip), jpiv.ctypes.data_as(_ip),
+            ctypes.byref(info), _stream_ptr(stream))
+    return Status(st), f, ipiv[:p], jpiv[:p], info.value
+
+
+def gemm_sub_device(M, N, K, c_ptr, a_ptr, b_ptr, ld, stream=None) -> Status:
+    return Status(lib().dlu_gemm_sub_device(M, N, K, c_ptr, a_ptr, b_ptr, ld,
+                                            _stream_ptr(stream)))
+
+
+# ---- torch plumbing: column-major device matrices -------------------------------------------
+
+def to_device(a: np.ndarray, lda: Optional[int] = None):
+    """Copy a host matrix to a column-major CUDA/HIP tensor; returns (storage tensor, lda).
+
+    The storage tensor has shape (n, lda): row j is column j of the matrix.
+    """
+    import torch
+    m, n = a.shape
+    lda = lda or max(1, m)
+    host = np.zeros((n, lda), dtype=np.float64)
+    host[:, :m] = np.asarray(a, dtype=np.float64).T
+    return torch.from_numpy(host).to("cuda"), lda
+
+
+def from_device(t, m: int) -> np.ndarray:
+    """Inverse of to_device: the m x n matrix held by storage tensor t (shape (n, lda))."""
+    return np.asfortranarray(t.cpu().numpy()[:, :m].T)

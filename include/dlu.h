@@ -1,0 +1,108 @@
+/*
+ * dlu.h -- C ABI of libdlu, the MI355X (gfx950) delayed-pivoting LU.
+ *
+ * Drop-in boundary for the reference's GPU entry points
+ *   delayed_lu_cuda/delayed_dgetrf_cuda.hpp:7-63 (status enum, device/host/native/hybrid entry
+ *   points, status string), whose semantics are kept:
+ *   - column-major storage, A(i,j) at a[i + j*lda], lda >= max(1,m);
+ *   - in-place compact output: unit-lower L multipliers below the diagonal of the eliminated
+ *     columns, U above, and the Schur complement of the delayed part in [info,m) x [info,n);
+ *   - ipiv/jpiv: 0-based swap histories of length min(m,n) ("at step k swap row k with row
+ *     ipiv[k]", likewise columns), identity for steps at and after info;
+ *   - info = min(m,n) when nothing is delayed, info = k when the active block at step k is
+ *     entirely zero, info = -1 whenever the returned status is not DLU_SUCCESS;
+ *   - column pivoting first (largest |a|, lowest row on ties); a zero (or NaN-only) column falls
+ *     back to the largest |a| of the whole active block (lowest column-major offset on ties);
+ *   - ipiv, jpiv and info are HOST memory in every variant; the call runs on `stream` and
+ *     synchronizes it before returning.
+ * Names carry no vendor prefix; the ordinal values of dlu_status match the reference enum.
+ * `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ */
+#ifndef DLU_H
+#define DLU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* delayed_lu_cuda/delayed_dgetrf_cuda.hpp:7-13 (success, invalid_argument, allocation_failed,
+ * cuda_failure, cublas_failure), same ordinals. */
+typedef enum dlu_status {
+  DLU_SUCCESS = 0,
+  DLU_INVALID_ARGUMENT = 1,
+  DLU_ALLOCATION_FAILED = 2,
+  DLU_RUNTIME_FAILURE = 3, /* a HIP runtime call failed */
+  DLU_KERNEL_FAILURE = 4   /* a kernel reported failure (e.g. a bounded wait expired) */
+} dlu_status;
+
+/* Replaces delayed_getrf_cuda_device (delayed_lu_cuda/delayed_dgetrf_cuda.hpp:24-26,
+ * delayed_dgetrf_cuda.cu:403-447).  d_a is device memory. */
+dlu_status dlu_getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv, int *info,
+                            void *stream);
+
+/* Replaces delayed_getrf_cuda_host (delayed_dgetrf_cuda.hpp:35-37, .cu:449-488): copies lda*n
+ * doubles to the device, factors there, copies the result back into a. */
+dlu_status dlu_getrf_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv, int *info,
+                          void *stream);
+
+/* Replace delayed_getrf_cuda_native_{device,host} (delayed_dgetrf_cuda.hpp:40-46): aliases. */
+dlu_status dlu_getrf_native_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
+                                   int *info, void *stream);
+dlu_status dlu_getrf_native_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                                 int *info, void *stream);
+
+/* Replace delayed_getrf_cuda_hybrid_{device,host} (delayed_dgetrf_cuda.hpp:55-61).  The
+ * reference's CPU-panel/GPU-update path is not reproduced: the panel runs on the GPU here, so
+ * these are aliases of the device path with the same signature and results. */
+dlu_status dlu_getrf_hybrid_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
+                                   int *info, void *stream);
+dlu_status dlu_getrf_hybrid_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                                 int *info, void *stream);
+
+/* Replaces delayed_getrf_cuda_status_string (delayed_dgetrf_cuda.hpp:63). */
+const char *dlu_status_string(dlu_status status);
+
+/* ---- Extensions with no counterpart in the reference ------------------------------------ */
+
+/* Tuning knobs.  Zero-initialised fields take the defaults. */
+typedef struct dlu_options {
+  int nb;          /* panel width: 64, 128 or 256 (default 256) */
+  int lookahead;   /* 1 (default): factor panel j+1 while panel j's far update runs; -1: off */
+  long long spin_limit_ticks; /* bound on any in-kernel wait, 100 MHz ticks (default 2e8 = 2 s) */
+  int profile;     /* nonzero: time every panel / solve / update launch with HIP events */
+} dlu_options;
+
+/* What happened during one factorization. */
+typedef struct dlu_report {
+  int panels;       /* panel kernels launched (restarts included) */
+  int fallbacks;    /* full-pivot fallbacks taken */
+  int near_ties;    /* column steps whose relative top-2 magnitude gap was below 1e-10 */
+  int exact_ties;   /* column steps whose two largest magnitudes were exactly equal */
+  double min_margin;/* smallest relative top-2 gap over the column steps (2.0 if none) */
+  /* filled when dlu_options.profile is set: summed launch durations (event-timed on the stream
+   * each kernel runs on) and the algorithmic flops of the trailing updates (2*M*N*K each) */
+  int update_launches;
+  double update_seconds;
+  double update_flops;
+  int panel_launches;
+  double panel_seconds;
+  int solve_launches;
+  double solve_seconds;
+} dlu_report;
+
+/* dlu_getrf_device with options and a report (either may be NULL).  `margins`, when not NULL,
+ * is a HOST array of min(m,n) doubles receiving each column step's relative top-2 gap (-1 for
+ * steps decided by a full-pivot fallback or never reached). */
+dlu_status dlu_getrf_device_ex(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
+                               int *info, void *stream, const dlu_options *options,
+                               dlu_report *report, double *margins);
+
+/* Kernel-level entry point of the trailing update: C(MxN) -= A(MxK) * B(KxN), column-major
+ * device matrices sharing leading dimension ld.  Asynchronous on `stream`. */
+dlu_status dlu_gemm_sub_device(int M, int N, int K, double *C, const double *A, const double *B,
+                               int ld, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DLU_H */

@@ -1,0 +1,184 @@
+"""CPU oracle for the delayed-pivoting LU -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` import this
+module, and only as the checker (or the timed CPU baseline).  The product library never loads it.
+
+It wraps ``oracle/_build/libdlu_oracle.so`` (built from ``dlu_oracle.c`` with gcc), which restates
+the reference's ``delayed_getrf`` (delayed_lu/delayed_dgetrf.cpp:129-251) and ``lapack_getrf``
+(delayed_lu/lapack_dgetrf.cpp:21-63) on top of a BLAS/LAPACK found at run time.
+"""
+from __future__ import annotations
+
+import ctypes
+import glob
+import os
+import subprocess
+import sys
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD_DIR = os.path.join(HERE, "_build")
+LIB_PATH = os.path.join(BUILD_DIR, "libdlu_oracle.so")
+SOURCES = [os.path.join(HERE, "dlu_oracle.c")]
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with gcc (no BLAS at link time; it is opened at run time)."""
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    newest = max(os.path.getmtime(s) for s in SOURCES + [os.path.join(HERE, "dlu_oracle.h")])
+    if not force and os.path.exists(LIB_PATH) and os.path.getmtime(LIB_PATH) >= newest:
+        return LIB_PATH
+    tmp = LIB_PATH + ".tmp"
+    cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
+           "-o", tmp] + SOURCES + ["-ldl", "-lm"]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+def _blas_candidates():
+    env = os.environ.get("DLU_ORACLE_BLAS")
+    if env:
+        path, _, prefix = env.partition(":")
+        yield path, prefix
+    for p in ("/opt/conda/lib/libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so"):
+        if os.path.exists(p):
+            yield p, ""
+    try:
+        import scipy  # noqa: F401  (only to locate its bundled OpenBLAS)
+        libs = os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs")
+        for p in sorted(glob.glob(os.path.join(libs, "libscipy_openblas-*.so"))):
+            yield p, "scipy_"
+    except ImportError:
+        pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    # MKL: share the GNU OpenMP runtime that torch already uses instead of loading libiomp5.
+    os.environ.setdefault("MKL_THREADING_LAYER", "GNU")
+    L = ctypes.CDLL(LIB_PATH)
+    L.dlu_oracle_init.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    L.dlu_oracle_init.restype = ctypes.c_int
+    L.dlu_oracle_blas_name.restype = ctypes.c_char_p
+    L.dlu_oracle_set_threads.argtypes = [ctypes.c_int]
+    L.dlu_oracle_set_threads.restype = ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.dlu_oracle_delayed_getrf.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip, ip,
+                                           ip, ctypes.c_int, dp, ip]
+    L.dlu_oracle_lapack_getrf.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip, ip]
+    L.dlu_oracle_unblocked_ld.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip, ip,
+                                          ip]
+    L.dlu_oracle_backward_error.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int,
+                                            ip, ip, ctypes.c_int, ctypes.c_int]
+    L.dlu_oracle_backward_error.restype = ctypes.c_double
+    errors = []
+    for path, prefix in _blas_candidates():
+        rc = L.dlu_oracle_init(path.encode(), prefix.encode())
+        if rc == 0:
+            break
+        errors.append(f"{path}: rc={rc}")
+    else:
+        raise RuntimeError("oracle: no usable BLAS/LAPACK found: " + "; ".join(errors))
+    _lib = L
+    return L
+
+
+def blas_name() -> str:
+    return lib().dlu_oracle_blas_name().decode()
+
+
+def set_threads(n: int) -> int:
+    return lib().dlu_oracle_set_threads(int(n))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if a is not None else None
+
+
+def _fortran(a: np.ndarray) -> np.ndarray:
+    return np.array(a, dtype=np.float64, order="F", copy=True)
+
+
+def delayed_getrf(a: np.ndarray, nb: int = 64, margins: bool = False):
+    """Restated delayed_getrf on a copy of ``a`` (m x n).
+
+    Returns ``(lu, ipiv, jpiv, info, extra)`` where ``extra`` holds ``fallbacks`` and, when
+    requested, the per-step column-pivot ``margins``.
+    """
+    f = _fortran(a)
+    m, n = f.shape
+    p = min(m, n)
+    ipiv = np.zeros(max(p, 1), dtype=np.int32)
+    jpiv = np.zeros(max(p, 1), dtype=np.int32)
+    info = ctypes.c_int(-1)
+    fb = ctypes.c_int(0)
+    marg = np.zeros(max(p, 1), dtype=np.float64) if margins else None
+    lib().dlu_oracle_delayed_getrf(m, n, _dp(f), max(1, m), _ip(ipiv), _ip(jpiv),
+                                   ctypes.byref(info), nb, _dp(marg), ctypes.byref(fb))
+    extra = {"fallbacks": fb.value}
+    if margins:
+        extra["margins"] = marg[:p]
+    return f, ipiv[:p], jpiv[:p], info.value, extra
+
+
+def lapack_getrf(a: np.ndarray):
+    f = _fortran(a)
+    m, n = f.shape
+    p = min(m, n)
+    ipiv = np.zeros(max(p, 1), dtype=np.int32)
+    info = ctypes.c_int(-1)
+    lib().dlu_oracle_lapack_getrf(m, n, _dp(f), max(1, m), _ip(ipiv), ctypes.byref(info))
+    return f, ipiv[:p], info.value
+
+
+def unblocked_ld(a: np.ndarray):
+    f = _fortran(a)
+    m, n = f.shape
+    p = min(m, n)
+    ipiv = np.zeros(max(p, 1), dtype=np.int32)
+    jpiv = np.zeros(max(p, 1), dtype=np.int32)
+    info = ctypes.c_int(-1)
+    lib().dlu_oracle_unblocked_ld(m, n, _dp(f), max(1, m), _ip(ipiv), _ip(jpiv),
+                                  ctypes.byref(info))
+    return f, ipiv[:p], jpiv[:p], info.value
+
+
+def backward_error(original: np.ndarray, factor: np.ndarray, ipiv, jpiv: Optional[np.ndarray],
+                   info: int, delayed: bool = True) -> float:
+    """||P A Q - Lbar Ubar||_F / (max(m,n) eps ||A||_F) of a compact factorization."""
+    o = _fortran(original)
+    f = _fortran(factor)
+    m, n = o.shape
+    ip = np.ascontiguousarray(ipiv, dtype=np.int32)
+    jp = np.ascontiguousarray(jpiv, dtype=np.int32) if jpiv is not None else None
+    return lib().dlu_oracle_backward_error(m, n, _dp(o), _dp(f), max(1, m), _ip(ip), _ip(jp),
+                                           int(info), 1 if delayed else 0)
+
+
+def replay(piv, count: int, size: int) -> np.ndarray:
+    """Order vector from a swap history: swap(order[k], order[piv[k]]) for k < count."""
+    order = np.arange(size)
+    for k in range(count):
+        t = order[k]
+        order[k] = order[piv[k]]
+        order[piv[k]] = t
+    return order
+
+
+if __name__ == "__main__":  # manual probe: python oracle/oracle.py
+    build(force="--force" in sys.argv)
+    print(blas_name())

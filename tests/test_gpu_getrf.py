@@ -1,0 +1,238 @@
+"""GPU parity of the HIP delayed LU against the oracle, through the C ABI.
+
+Bars: ipiv / jpiv / info / delayed set bit-exact versus the CPU oracle
+(nb = 64, the reference's blocking) unless a near-tie is reported at the first differing step;
+factors within 1e-12 relative (max-norm, relative to max |factor|); scaled backward error
+||PAQ - LU||_F / (max(m,n) eps ||A||_F) <= 10.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import delayed_set, dlu, golden, oracle, reconstruction_error, workloads
+
+pytestmark = pytest.mark.gpu
+
+FACTOR_RTOL = 1e-12
+BWD_MAX = 10.0
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    dlu.lib()  # raises when libdlu.so is missing
+    return torch
+
+
+def gpu_factor(a, nb=None, lda=None, lookahead=None, margins=True):
+    torch = _torch()
+    m, n = a.shape
+    t, lda = dlu.to_device(a, lda)
+    opts = {}
+    if nb:
+        opts["nb"] = nb
+    if lookahead is not None:
+        opts["lookahead"] = lookahead
+    st, ipiv, jpiv, info, rep, marg = dlu.getrf_device(m, n, t.data_ptr(), lda, options=opts,
+                                                        margins=margins)
+    torch.cuda.synchronize()
+    assert st == dlu.Status.SUCCESS, dlu.status_string(st)
+    return dlu.from_device(t, m), ipiv, jpiv, info, rep, marg
+
+
+def check_parity(a, lu, ipiv, jpiv, info, marg, nb_cpu=64):
+    lu_c, ip_c, jp_c, info_c, ex = oracle.delayed_getrf(a, nb=nb_cpu, margins=True)
+    same = info == info_c and np.array_equal(ipiv, ip_c) and np.array_equal(jpiv, jp_c)
+    if not same:
+        k = int(np.argmax((ipiv != ip_c) | (jpiv != jp_c)))
+        gm = marg[k] if marg is not None else None
+        cm = ex["margins"][k]
+        pytest.fail(f"pivots differ first at step {k}: gpu ({ipiv[k]},{jpiv[k]}) cpu "
+                    f"({ip_c[k]},{jp_c[k]}); margins gpu={gm} cpu={cm}; info {info} vs {info_c}")
+    scale = max(1.0, np.abs(lu_c).max())
+    assert np.abs(lu - lu_c).max() <= FACTOR_RTOL * scale
+    bwd = oracle.backward_error(a, lu, ipiv, jpiv, info)
+    assert 0 <= bwd <= BWD_MAX
+    return bwd
+
+
+def _case_matrix(c):
+    a = np.zeros((c["m"], c["n"]), order="F")
+    if "dense_rows" in c:
+        a[:, :] = np.array(c["dense_rows"])
+    for i, j, v in c.get("entries", []):
+        a[i, j] = v
+    return a
+
+
+@pytest.mark.parametrize("M,N,K", [(16, 16, 4), (64, 64, 16), (100, 37, 13), (513, 260, 256),
+                                   (1024, 1024, 256), (2000, 3000, 64), (128, 128, 1)])
+def test_gemm_sub_matches_numpy(M, N, K):
+    torch = _torch()
+    rng = np.random.default_rng(M * 7 + N + K)
+    ld = M + 3
+    c = rng.standard_normal((M, N))
+    a = rng.standard_normal((M, K))
+    b = rng.standard_normal((K, N))
+    tc, _ = dlu.to_device(c, ld)
+    ta, _ = dlu.to_device(a, ld)
+    tb, _ = dlu.to_device(b, ld)
+    st = dlu.gemm_sub_device(M, N, K, tc.data_ptr(), ta.data_ptr(), tb.data_ptr(), ld)
+    torch.cuda.synchronize()
+    assert st == dlu.Status.SUCCESS
+    got = dlu.from_device(tc, M)
+    ref = c - a @ b
+    assert np.abs(got - ref).max() <= 1e-13 * K * max(1.0, np.abs(ref).max())
+    # padding rows between M and ld untouched
+    assert np.all(tc.cpu().numpy()[:, M:] == 0.0)
+
+
+@pytest.mark.parametrize("idx", range(4))
+@pytest.mark.parametrize("variant", ["host", "native", "hybrid"])
+def test_known_answers_host_interface(idx, variant):
+    _torch()
+    c = golden("known_answers.json")["cases"][idx]
+    a = _case_matrix(c)
+    st, lu, ipiv, jpiv, info = dlu.getrf_host(a, variant=variant)
+    assert st == dlu.Status.SUCCESS
+    assert info == c["info"]
+    assert ipiv.tolist() == c["ipiv"] and jpiv.tolist() == c["jpiv"]
+    tol = 1e-11 * max(1.0, np.abs(a).max()) * max(a.shape)  # example.cu:101-106
+    assert reconstruction_error(a, lu, ipiv, jpiv, info) <= tol
+
+
+def test_known_answer_example_cpp_device():
+    c = golden("known_answers.json")["cases"][4]
+    a = _case_matrix(c)
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a)
+    assert info == 800
+    assert ipiv.tolist() == c["ipiv"] and jpiv.tolist() == c["jpiv"]
+    assert rep["fallbacks"] == 800
+    # the delayed Schur complement is the untouched zero block
+    assert np.all(lu[800:, 800:] == 0.0)
+
+
+@pytest.mark.parametrize("n,nb", [(1, 256), (2, 256), (17, 64), (64, 64), (100, 128),
+                                  (257, 256), (512, 64), (512, 256), (1000, 128), (2048, 256)])
+def test_normal_parity(n, nb):
+    a = workloads.normal(n, n, 1000 + n)
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a, nb=nb)
+    check_parity(a, lu, ipiv, jpiv, info, marg)
+    assert rep["fallbacks"] == 0
+
+
+@pytest.mark.parametrize("m,n", [(300, 200), (200, 300), (1000, 64), (64, 1000), (777, 513)])
+def test_rectangular_parity(m, n):
+    a = workloads.normal(m, n, m * 31 + n)
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a, nb=128)
+    check_parity(a, lu, ipiv, jpiv, info, marg)
+
+
+def test_leading_dimension_padding():
+    a = workloads.normal(300, 280, 9)
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a, lda=333)
+    check_parity(a, lu, ipiv, jpiv, info, marg)
+
+
+@pytest.mark.parametrize("n,seed,nb", [(512, 3, 256), (1024, 5, 256), (1024, 5, 64),
+                                       (2048, 11, 256)])
+def test_delay_stress_parity(n, seed, nb):
+    a, zc, tc, off = workloads.delay_stress(n, seed)
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a, nb=nb)
+    assert info == n - len(zc)
+    assert np.array_equal(delayed_set(jpiv, info, n), zc)
+    assert np.all(lu[info:, info:] == 0.0)
+    check_parity(a, lu, ipiv, jpiv, info, marg)
+
+
+@pytest.mark.parametrize("cols", [[0], [5], [100, 101], [255, 256, 257], [300, 10, 700]])
+def test_zero_columns_mid_panel(cols):
+    n = 768
+    a = workloads.normal(n, n, 77 + sum(cols))
+    a[:, cols] = 0.0
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a, nb=256)
+    assert info == n - len(cols)
+    assert np.array_equal(delayed_set(jpiv, info, n), np.sort(cols))
+    check_parity(a, lu, ipiv, jpiv, info, marg)
+
+
+def test_oracle_golden_vectors_on_gpu():
+    d = golden("oracle_vectors.json")
+    for c in d["cases"]:
+        if c["kind"] == "normal":
+            a = workloads.normal(c["m"], c["n"], c["seed"])
+        else:
+            a = workloads.delay_stress(c["n"], c["seed"])[0]
+        print("case", c["name"], flush=True)
+        lu, ipiv, jpiv, info, rep, marg = gpu_factor(a)
+        assert info == c["info"], c["name"]
+        assert ipiv.tolist() == c["ipiv"], c["name"]
+        assert jpiv.tolist() == c["jpiv"], c["name"]
+
+
+def test_all_zero_and_rank_one():
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(np.zeros((300, 300), order="F"))
+    assert info == 0 and ipiv.tolist() == list(range(300))
+    # powers of two keep every product, reciprocal and difference exact: the Schur complement
+    # after one step is exactly zero, so the rest is delayed
+    x = 2.0 ** (np.arange(300) % 10)
+    y = 2.0 ** (np.arange(300) % 7 - 3)
+    a = np.asfortranarray(np.outer(x, y))
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a)
+    check_parity(a, lu, ipiv, jpiv, info, marg)
+    assert info == 1
+
+
+def test_nan_column_is_never_a_pivot():
+    """GPU rule (delayed_lu_cuda/delayed_dgetrf_cuda.cu:46): NaN never wins a comparison, so an
+    all-NaN column is treated like a zero column and ends up delayed.  The CPU reference depends
+    on its BLAS idamax here, so this is checked on the GPU alone."""
+    n = 200
+    a = workloads.normal(n, n, 5)
+    a[:, 37] = np.nan
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a)
+    assert info == n - 1
+    assert delayed_set(jpiv, info, n).tolist() == [37]
+    assert np.isfinite(lu[:info, :info]).all()
+
+
+def test_host_equals_device_and_deterministic():
+    a = workloads.normal(700, 700, 21)
+    lu1, ip1, jp1, info1, _, _ = gpu_factor(a, margins=False)
+    lu2, ip2, jp2, info2, _, _ = gpu_factor(a, margins=False)
+    st, lu3, ip3, jp3, info3 = dlu.getrf_host(a)
+    assert st == dlu.Status.SUCCESS
+    assert np.array_equal(lu1, lu2) and np.array_equal(lu1, lu3)
+    assert np.array_equal(ip1, ip3) and np.array_equal(jp1, jp3) and info1 == info3
+
+
+def test_lookahead_off_gives_same_result():
+    a = workloads.normal(1500, 1500, 33)
+    on = gpu_factor(a, nb=256)
+    off = gpu_factor(a, nb=256, lookahead=-1)
+    assert np.array_equal(on[1], off[1]) and np.array_equal(on[2], off[2])
+    assert np.abs(on[0] - off[0]).max() <= 1e-13 * np.abs(on[0]).max()
+
+
+def test_margins_report_matches_oracle():
+    a = workloads.normal(512, 512, 1)
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a)
+    _, _, _, _, ex = oracle.delayed_getrf(a, margins=True)
+    assert rep["exact_ties"] == 0
+    assert abs(rep["min_margin"] - ex["margins"].min()) <= 1e-9
+    assert np.allclose(marg, ex["margins"], atol=1e-9)
+
+
+@pytest.mark.parametrize("n", [4096, 8192])
+def test_large_normal_parity(n):
+    a = workloads.normal(n, n, 20260000 + n)
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a, nb=256)
+    print(f"n={n} panels={rep['panels']} min_margin={rep['min_margin']:.3e}", flush=True)
+    lu_c, ip_c, jp_c, info_c, ex = oracle.delayed_getrf(a, nb=64)
+    assert info == info_c == n
+    if not np.array_equal(ipiv, ip_c):
+        k = int(np.argmax(ipiv != ip_c))
+        assert marg[k] < 1e-10, f"step {k} differs with margin {marg[k]}"
+    assert np.array_equal(jpiv, jp_c)
+    assert np.abs(lu - lu_c).max() <= FACTOR_RTOL * np.abs(lu_c).max()
