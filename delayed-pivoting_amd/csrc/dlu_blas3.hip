@@ -23,7 +23,7 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // v_mfma_f64_16x16x4_f64 operand layout (lane l of 64):
 //   A (16 x 4): one double, row l % 16, k l / 16
 //   B (4 x 16): one double, k l / 16, column l % 16
-//   D (16 x 16): four doubles r = 0..3, row 4 * (l / 16) + r, column l % 16
+//   D (16 x 16): four doubles r = 0..3, row l / 16 + 4 * r, column l % 16
 // ---------------------------------------------------------------------------------------------
 template <int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_sub_kernel(double *__restrict__ C,
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(double *__restrict__ C,
       if (col >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = row0 + wm * WM + 16 * i + 4 * (lane >> 4) + r;
+        const int row = row0 + wm * WM + 16 * i + (lane >> 4) + 4 * r;
         if (row < M) {
           double *cp = C + static_cast<size_t>(row) + static_cast<size_t>(col) * lda;
           *cp = *cp - acc[i][j][r];

@@ -61,6 +61,7 @@ struct Workspace {
   int *d_lists = nullptr;
   double *d_pv = nullptr;
   unsigned long long *d_po = nullptr;
+  unsigned long long *d_phase = nullptr;
   unsigned seq = 0;
   hipStream_t aux = nullptr;
   hipEvent_t ev_ctl = nullptr, ev_aux = nullptr, ev_start = nullptr;
@@ -92,6 +93,7 @@ hipError_t prepare(Workspace &w, int device, int steps) {
     if ((e = hipMalloc(&w.d_lists, 2 * kListStride * sizeof(int))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_pv, kSearchPartials * sizeof(double))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_po, kSearchPartials * sizeof(unsigned long long))) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_phase, 8 * sizeof(unsigned long long))) != hipSuccess) return e;
     if ((e = hipStreamCreateWithFlags(&w.aux, hipStreamNonBlocking)) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&w.ev_ctl, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&w.ev_aux, hipEventDisableTiming)) != hipSuccess) return e;
@@ -193,6 +195,7 @@ struct Run {
     p.rowk = w.d_rowk;
     p.swap_list = w.d_lists + parity * kListStride;
     p.spin_limit = spin;
+    p.phase_ticks = profile ? w.d_phase : nullptr;
     hipError_t e = profile ? mark(t_panel, s, 0.0) : hipSuccess;
     if (e == hipSuccess) e = launch_panel(p, nb, rpl, s);
     if (e == hipSuccess && profile) e = close(t_panel, s);
@@ -342,6 +345,11 @@ dlu_status getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv
   }
   Run run{w, m, n, lda, steps, nb, d_a, stream, spin};
   run.profile = opt && opt->profile;
+  if (run.profile && (e = hipMemsetAsync(w.d_phase, 0, 8 * sizeof(unsigned long long), stream)) !=
+                         hipSuccess) {
+    *info = -1;
+    return to_status(e);
+  }
   {
     int probe_groups = 0, probe_rpl = 0;
     if (!run.panel_shape(0, probe_groups, probe_rpl)) {
@@ -389,6 +397,9 @@ dlu_status getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv
       rep->panel_seconds = Run::total(run.t_panel, nullptr);
       rep->solve_launches = static_cast<int>(run.t_solve.size());
       rep->solve_seconds = Run::total(run.t_solve, nullptr);
+      unsigned long long ticks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (hipMemcpy(ticks, w.d_phase, sizeof ticks, hipMemcpyDeviceToHost) == hipSuccess)
+        for (int i = 0; i < 8; ++i) rep->panel_phase_seconds[i] = ticks[i] * 1e-8;  // 100 MHz
     }
   }
   run.release();

@@ -58,6 +58,7 @@ struct PanelParams {
   uint64_t *rowk;      // [2][2*nb]
   int *swap_list;      // swap_list_ints(nb) ints for this panel (parity selected by the host)
   long long spin_limit;  // bounded wait, in s_memrealtime ticks (100 MHz)
+  unsigned long long *phase_ticks;  // optional [8] per-phase tick totals (workgroup 0)
 };
 
 // Launchers (return hipSuccess or the launch error).

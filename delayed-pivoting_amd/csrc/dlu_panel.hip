@@ -164,6 +164,16 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
   int zero_col = 0;
   double min_margin = 2.0;
   int near_ties = 0, exact_ties = 0;
+  // Optional phase timing (workgroup 0, thread 0; 100 MHz ticks).
+  const bool timed = P.phase_ticks != nullptr && g == 0 && threadIdx.x == 0;
+  unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long tp = timed ? wall_clock64() : 0;
+#define DLU_PHASE(i)                         \
+  if (timed) {                               \
+    const unsigned long long tn = wall_clock64(); \
+    ph[i] += tn - tp;                        \
+    tp = tn;                                 \
+  }
 
   for (int c = 0; c < width; ++c) {
     const int k = P.pb + c;
@@ -200,6 +210,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       }
     }
     __syncthreads();
+    DLU_PHASE(0)
     const int lrow = s_loc_row;
 
     // Publish: header, candidate row, and row k when this workgroup owns it.
@@ -227,6 +238,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       }
     }
 
+    DLU_PHASE(1)
     // Gather every workgroup's header.
     if (threadIdx.x < P.groups) {
       const uint64_t *hs = P.slots + (static_cast<size_t>(par) * P.groups + threadIdx.x) * SW;
@@ -237,6 +249,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       s_crow[threadIdx.x] = static_cast<int>(w[4]);
     }
     __syncthreads();
+    DLU_PHASE(2)
     if (s_fail) goto leave;
 
     {
@@ -256,6 +269,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
         }
       }
       wave_top2(g1, grow, g2, gwin);
+      DLU_PHASE(3)
 
       if (!(g1 > 0.0)) {  // zero (or NaN-only) column: stop the panel at k
         stop = k;
@@ -277,6 +291,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
         }
       }
       wave_sync_lds();
+      DLU_PHASE(4)
 
       // Interchange rows k and p on this workgroup's rows.
 #pragma unroll
@@ -306,11 +321,11 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
         }
       }
       if (g == 0 && threadIdx.x == 0) {
-        P.ipiv[k] = p;
+        if (!forced) P.ipiv[k] = p;  // a forced step keeps the fallback's recorded row
         s_hist[c] = p;
         double margin = -1.0;
         if (!forced) {
-          margin = (g1 - g2) / g1;
+          margin = (g1 - fmax(g2, 0.0)) / g1;
           min_margin = fmin(min_margin, margin);
           if (margin == 0.0) ++exact_ties;
           if (margin < 1e-10) ++near_ties;
@@ -318,6 +333,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
         if (P.margins) P.margins[k] = margin;
       }
       __syncthreads();
+      DLU_PHASE(5)
       if (s_fail) goto leave;
 
       // Rank-1 update of this workgroup's rows, panel columns right of c.
@@ -331,6 +347,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
             if (col0 + j > c && col0 + j < width) v[r][j] = fma(-l, s_urow[col0 + j], v[r][j]);
         }
       }
+      DLU_PHASE(6)
     }
   }
 
@@ -410,6 +427,10 @@ finish:
             ++cnt;
           }
         P.swap_list[4 * NB] = cnt;
+        if (timed) {
+          DLU_PHASE(7)
+          for (int i = 0; i < 8; ++i) P.phase_ticks[i] += ph[i];
+        }
         P.ctl->stop = stop;
         P.ctl->zero_column = zero_col;
         P.ctl->near_ties += near_ties;
@@ -419,6 +440,7 @@ finish:
     }
   }
   return;
+#undef DLU_PHASE
 
 leave:
   if (g == 0 && threadIdx.x == 0) P.ctl->stop = P.pb;

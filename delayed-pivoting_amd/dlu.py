@@ -39,10 +39,13 @@ class Report(ctypes.Structure):
                 ("update_launches", ctypes.c_int), ("update_seconds", ctypes.c_double),
                 ("update_flops", ctypes.c_double), ("panel_launches", ctypes.c_int),
                 ("panel_seconds", ctypes.c_double), ("solve_launches", ctypes.c_int),
-                ("solve_seconds", ctypes.c_double)]
+                ("solve_seconds", ctypes.c_double),
+                ("panel_phase_seconds", ctypes.c_double * 8)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["panel_phase_seconds"] = list(self.panel_phase_seconds)
+        return d
 
 
 _lib = None

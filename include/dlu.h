@@ -88,6 +88,9 @@ typedef struct dlu_report {
   double panel_seconds;
   int solve_launches;
   double solve_seconds;
+  /* panel kernel, workgroup 0, summed over all steps (profile only): local argmax, publish,
+   * candidate exchange, reduction, pivot-row fetch, interchange+scale, rank-1 update, tail */
+  double panel_phase_seconds[8];
 } dlu_report;
 
 /* dlu_getrf_device with options and a report (either may be NULL).  `margins`, when not NULL,
