@@ -17,26 +17,32 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 //
 // Workgroup tile BM x BN, 256 threads = 4 waves in a 2 x 2 grid, each wave owns a
 // (BM/2) x (BN/2) block made of 16 x 16 v_mfma_f64_16x16x4_f64 tiles.  K advances in chunks of
-// 16 staged through double-buffered LDS; the next chunk's global loads are in flight while the
-// current chunk feeds the MFMA pipe.
+// BK staged through double-buffered LDS; the next chunk's global loads are in flight while the
+// current chunk feeds the MFMA pipe.  Global loads are lane-contiguous (consecutive lanes read
+// consecutive rows of A, consecutive k of B), 16 bytes per lane when the operands allow it.
 //
 // v_mfma_f64_16x16x4_f64 operand layout (lane l of 64):
 //   A (16 x 4): one double, row l % 16, k l / 16
 //   B (4 x 16): one double, k l / 16, column l % 16
 //   D (16 x 16): four doubles r = 0..3, row l / 16 + 4 * r, column l % 16
 // ---------------------------------------------------------------------------------------------
-template <int BM, int BN>
-__global__ __launch_bounds__(256) void gemm_sub_kernel(double *__restrict__ C,
-                                                      const double *__restrict__ A,
-                                                      const double *__restrict__ B, int lda,
-                                                      int M, int N, int K, int tiles_m,
-                                                      int tiles_n) {
-  constexpr int BK = 16;
+template <int BM, int BN, int BK, bool VEC>
+__global__ __launch_bounds__(256, 2) void gemm_sub_kernel(double *__restrict__ C,
+                                                         const double *__restrict__ A,
+                                                         const double *__restrict__ B, int lda,
+                                                         int M, int N, int K, int tiles_m,
+                                                         int tiles_n) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int BKP = BK + 2;  // padded k stride of the B tile (16-byte aligned rows)
-  __shared__ double As[2][BK][BM];
-  __shared__ double Bs[2][BN][BKP];
+  constexpr int AS = BM + 16;  // A tile k-row stride: consecutive k-rows a fragment read touches
+                               // land on opposite LDS bank halves
+  constexpr int BS = BK + 2;   // B tile column stride (16-byte aligned, conflict-free reads)
+  constexpr int VW = VEC ? 2 : 1;             // doubles per global load
+  constexpr int A_LD = BM * BK / (256 * VW);  // loads per thread per chunk
+  constexpr int B_LD = BN * BK / (256 * VW);
+  constexpr int KS = BK / 4;                  // MFMA k-steps per chunk
+  __shared__ double As[2][BK][AS];
+  __shared__ double Bs[2][BN][BS];
 
   // Tile schedule.  Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
   // range of tiles, then walk that range in groups of 8 tile rows so neighbours share L2 data.
@@ -52,37 +58,47 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(double *__restrict__ C,
   const int tn = (tid % in_group) / gsize;
   const int row0 = tm * BM, col0 = tn * BN;
 
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
   const int wm = wave & 1, wn = wave >> 1;
 
-  constexpr int A_PER = BM * BK / 256;  // doubles loaded per thread
-  constexpr int A_TPC = BM / A_PER;     // threads per A column
-  constexpr int B_PER = BN * BK / 256;
-  constexpr int B_TPC = BK / B_PER;
-  const int a_k = threadIdx.x / A_TPC, a_r = (threadIdx.x % A_TPC) * A_PER;
-  const int b_n = threadIdx.x / B_TPC, b_k = (threadIdx.x % B_TPC) * B_PER;
-
-  double ra[A_PER], rb[B_PER];
+  // Global loads are branch-free: an out-of-range element is read from a clamped in-range
+  // address and replaced by zero.  On the vector path M and K are even, so a pair is either
+  // wholly inside or wholly outside.
+  typedef double dv_t __attribute__((ext_vector_type(VW)));
+  dv_t ra[A_LD], rb[B_LD];
   auto load = [&](int k0) {
-    const int ka = k0 + a_k;
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i) {
-      const int row = row0 + a_r + i;
-      ra[i] = (row < M && ka < K) ? A[static_cast<size_t>(row) + static_cast<size_t>(ka) * lda] : 0.0;
+    for (int i = 0; i < A_LD; ++i) {
+      const int e = t + 256 * i;
+      const int row = row0 + (e % (BM / VW)) * VW, kk = k0 + e / (BM / VW);
+      const bool ok = row < M && kk < K;
+      const dv_t val = *reinterpret_cast<const dv_t *>(
+          A + static_cast<size_t>(ok ? row : 0) + static_cast<size_t>(ok ? kk : 0) * lda);
+      ra[i] = ok ? val : dv_t(0.0);
     }
-    const int col = col0 + b_n;
 #pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      const int kb = k0 + b_k + i;
-      rb[i] = (kb < K && col < N) ? B[static_cast<size_t>(kb) + static_cast<size_t>(col) * lda] : 0.0;
+    for (int i = 0; i < B_LD; ++i) {
+      const int e = t + 256 * i;
+      const int kk = k0 + (e % (BK / VW)) * VW, col = col0 + e / (BK / VW);
+      const bool ok = col < N && kk < K;
+      const dv_t val = *reinterpret_cast<const dv_t *>(
+          B + static_cast<size_t>(ok ? kk : 0) + static_cast<size_t>(ok ? col : 0) * lda);
+      rb[i] = ok ? val : dv_t(0.0);
     }
   };
   auto stash = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i) As[buf][a_k][a_r + i] = ra[i];
+    for (int i = 0; i < A_LD; ++i) {
+      const int e = t + 256 * i;
+      *reinterpret_cast<dv_t *>(&As[buf][e / (BM / VW)][(e % (BM / VW)) * VW]) = ra[i];
+    }
 #pragma unroll
-    for (int i = 0; i < B_PER; ++i) Bs[buf][b_n][b_k + i] = rb[i];
+    for (int i = 0; i < B_LD; ++i) {
+      const int e = t + 256 * i;
+      *reinterpret_cast<dv_t *>(&Bs[buf][e / (BK / VW)][(e % (BK / VW)) * VW]) = rb[i];
+    }
   };
 
   double4_t acc[FM][FN];
@@ -91,6 +107,16 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(double *__restrict__ C,
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
 
+  double af[2][FM], bf[2][FN];
+  auto frags = [&](int buf, int kk, int slot) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+      af[slot][i] = As[buf][kk + (lane >> 4)][wm * WM + 16 * i + (lane & 15)];
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+      bf[slot][j] = Bs[buf][wn * WN + 16 * j + (lane & 15)][kk + (lane >> 4)];
+  };
+
   const int nk = (K + BK - 1) / BK;
   load(0);
   stash(0);
@@ -98,18 +124,19 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(double *__restrict__ C,
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) load((kt + 1) * BK);
+    frags(cur, 0, 0);
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      double af[FM], bf[FN];
-#pragma unroll
-      for (int i = 0; i < FM; ++i) af[i] = As[cur][kk + (lane >> 4)][wm * WM + 16 * i + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < FN; ++j) bf[j] = Bs[cur][wn * WN + 16 * j + (lane & 15)][kk + (lane >> 4)];
+    for (int s = 0; s < KS; ++s) {
+      // LDS reads for the next k-step are issued before this step's MFMAs.
+      if (s + 1 < KS) frags(cur, 4 * (s + 1), (s + 1) & 1);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s & 1][i], bf[s & 1][j], acc[i][j],
+                                                          0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
     }
     if (kt + 1 < nk) stash(cur ^ 1);
     __syncthreads();
@@ -133,31 +160,50 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(double *__restrict__ C,
 }
 
 // ---------------------------------------------------------------------------------------------
-// Row interchange + unit-lower triangular solve on a strip of NS columns.
+// Row interchange + unit-lower triangular solve on a strip of 16 columns.
 //
 // swap_list holds the panel's net row movement as (dst, src) pairs: entries [0, w) are rows
 // pb .. pb+w-1 in order, later entries are rows >= pb+w that received a row from the panel.
 // The strip's touched rows are gathered (all reads before any write), displaced rows are written
 // back in place, and rows [pb, pb+w) are solved in LDS:
 //   X := L11^{-1} X  with L11 = A(pb:pb+w, pb:pb+w) unit lower,
-// by 16-row blocks: the diagonal 16 x 16 blocks are inverted explicitly (every workgroup does
-// it for itself), then X_b := inv(L_bb) X_b and X_rest -= L(rest, b) X_b.
+// by 16-row blocks on FP64 MFMA: X_b := inv(L_bb) X_b (the 16 x 16 diagonal inverses are formed
+// by every workgroup for itself), then X_rest -= L(rest, b) X_b with L's column slice staged in
+// LDS.  Rows at and beyond w are zero-padded and never written back.
 // ---------------------------------------------------------------------------------------------
-template <int NB, int NS>
+template <int NB>
 __global__ __launch_bounds__(256) void swap_trsm_kernel(double *__restrict__ a, int lda, int pb,
                                                        int w, int c0, int c1,
                                                        const int *__restrict__ swap_list) {
-  __shared__ double X[NS][NB + 1];
-  __shared__ double E[NS][NB + 1];
+  constexpr int NS = 16;
+  constexpr int XS = NS + 2;  // row stride of X (doubles)
+  __shared__ double X[NB][XS];
+  __shared__ double E[NB][NS + 1];
   __shared__ double invL[NB / 16][16][17];
+  __shared__ double Ls[NB][17];  // rows [r0+16, w) of L's current 16-column slice
   const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
   const int cnt = swap_list[4 * NB];
+  const int nx = cnt - w;  // displaced rows below the panel
   const int *dst = swap_list;
   const int *src = swap_list + 2 * NB;
   const int cs = c0 + blockIdx.x * NS;
   const int ncol = (c1 - cs) < NS ? (c1 - cs) : NS;
   const int nblk = (w + 15) / 16;
   const double *L = a + static_cast<size_t>(pb) + static_cast<size_t>(pb) * lda;
+
+  // Gather every touched row of the strip before anything is written (rows are contiguous in
+  // lane order for a fixed column).
+  for (int idx = t; idx < cnt * NS; idx += 256) {
+    const int i = idx % cnt, col = idx / cnt;
+    const double val =
+        col < ncol ? a[static_cast<size_t>(src[i]) + static_cast<size_t>(cs + col) * lda] : 0.0;
+    if (i < w)
+      X[i][col] = val;
+    else
+      E[i - w][col] = val;
+  }
+  for (int idx = t; idx < (nblk * 16 - w) * NS; idx += 256) X[w + idx / NS][idx % NS] = 0.0;
 
   // Inverses of the diagonal blocks: thread (blk, col) solves L_bb x = e_col.
   if (t < 16 * nblk) {
@@ -167,75 +213,86 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(double *__restrict__ a, 
     double x[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      double s = (r == col) ? 1.0 : 0.0;
+      double sum = (r == col) ? 1.0 : 0.0;
       if (r > col && r < nr) {
 #pragma unroll
         for (int q = 0; q < 16; ++q)
           if (q >= col && q < r)
-            s -= L[static_cast<size_t>(r0 + r) + static_cast<size_t>(r0 + q) * lda] * x[q];
+            sum -= L[static_cast<size_t>(r0 + r) + static_cast<size_t>(r0 + q) * lda] * x[q];
       }
-      x[r] = (r < nr) ? s : 0.0;
+      x[r] = (r < nr) ? sum : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) invL[blk][r][col] = x[r];
   }
-
-  // Gather every touched row of the strip before anything is written.
-  for (int idx = t; idx < cnt * ncol; idx += 256) {
-    const int i = idx % cnt, col = idx / cnt;
-    const double val = a[static_cast<size_t>(src[i]) + static_cast<size_t>(cs + col) * lda];
-    if (i < w)
-      X[col][i] = val;
-    else
-      E[col][i - w] = val;
-  }
   __syncthreads();
-  for (int idx = t; idx < (cnt - w) * ncol; idx += 256) {
-    const int i = idx % (cnt - w), col = idx / (cnt - w);
-    a[static_cast<size_t>(dst[w + i]) + static_cast<size_t>(cs + col) * lda] = E[col][i];
+  for (int idx = t; idx < nx * ncol; idx += 256) {
+    const int i = idx % nx, col = idx / nx;
+    a[static_cast<size_t>(dst[w + i]) + static_cast<size_t>(cs + col) * lda] = E[i][col];
   }
 
+  const int lr = lane & 15, lk = lane >> 4;
   for (int blk = 0; blk < nblk; ++blk) {
     const int r0 = blk * 16;
-    const int nr = (w - r0) < 16 ? (w - r0) : 16;
-    // X_b := inv(L_bb) X_b
-    double y = 0.0;
-    const int rr = t & 15, cc = t >> 4;  // 16 rows x 16 columns of the strip
-    if (cc < ncol && rr < nr) {
+    const int below = nblk * 16 - (r0 + 16);  // padded rows under the block
+    // Stage L(r0+16 .. w, r0 .. r0+16) (zero beyond w) while wave 0 solves the diagonal block.
+    for (int idx = t; idx < below * 16; idx += 256) {
+      const int rr = idx % below, q = idx / below;
+      const int row = r0 + 16 + rr, colq = r0 + q;
+      Ls[rr][q] = (row < w && colq < w)
+                      ? L[static_cast<size_t>(row) + static_cast<size_t>(colq) * lda]
+                      : 0.0;
+    }
+    if (wave == 0) {
+      double4_t d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int q = 0; q < 16; ++q)
-        if (q <= rr && q < nr) y += invL[blk][rr][q] * X[cc][r0 + q];
+      for (int k4 = 0; k4 < 16; k4 += 4) {
+        const double av = invL[blk][lr][k4 + lk];
+        const double bv = X[r0 + k4 + lk][lr];
+        d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) X[r0 + lk + 4 * r][lr] = d[r];
     }
     __syncthreads();
-    if (cc < ncol && rr < nr) X[cc][r0 + rr] = y;
-    __syncthreads();
-    // rows below the block: X_r -= L(r, b) X_b
-    const int col = t % NS, rg = t / NS;
-    constexpr int RG = 256 / NS;
-    if (col < ncol) {
-      for (int r = r0 + 16 + rg; r < w; r += RG) {
-        double s = X[col][r];
+    // X(rows below) -= L(rows below, block) * X_b, one 16 x 16 tile per wave at a time.
+    for (int tb = wave; tb < below / 16; tb += 4) {
+      const int rt = r0 + 16 + 16 * tb;
+      double4_t d;
 #pragma unroll
-        for (int q = 0; q < 16; ++q)
-          if (q < nr) s -= L[static_cast<size_t>(r) + static_cast<size_t>(r0 + q) * lda] * X[col][r0 + q];
-        X[col][r] = s;
+      for (int r = 0; r < 4; ++r) d[r] = X[rt + lk + 4 * r][lr];
+#pragma unroll
+      for (int k4 = 0; k4 < 16; k4 += 4) {
+        const double av = -Ls[16 * tb + lr][k4 + lk];
+        const double bv = X[r0 + k4 + lk][lr];
+        d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
       }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) X[rt + lk + 4 * r][lr] = d[r];
     }
     __syncthreads();
   }
 
   for (int idx = t; idx < w * ncol; idx += 256) {
     const int i = idx % w, col = idx / w;
-    a[static_cast<size_t>(pb + i) + static_cast<size_t>(cs + col) * lda] = X[col][i];
+    a[static_cast<size_t>(pb + i) + static_cast<size_t>(cs + col) * lda] = X[i][col];
   }
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int BK>
 hipError_t launch_gemm_tile(double *c, const double *a, const double *b, int lda, int M, int N,
                             int K, hipStream_t s) {
   const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
-  hipLaunchKernelGGL((gemm_sub_kernel<BM, BN>), dim3(tm * tn), dim3(256), 0, s, c, a, b, lda, M,
-                     N, K, tm, tn);
+  // 16-byte loads: 16-byte aligned origins, even leading dimension, even M and K.
+  const bool vec = (lda % 2 == 0) && (M % 2 == 0) && (K % 2 == 0) &&
+                   (reinterpret_cast<uintptr_t>(a) % 16 == 0) &&
+                   (reinterpret_cast<uintptr_t>(b) % 16 == 0);
+  if (vec)
+    hipLaunchKernelGGL((gemm_sub_kernel<BM, BN, BK, true>), dim3(tm * tn), dim3(256), 0, s, c, a,
+                       b, lda, M, N, K, tm, tn);
+  else
+    hipLaunchKernelGGL((gemm_sub_kernel<BM, BN, BK, false>), dim3(tm * tn), dim3(256), 0, s, c, a,
+                       b, lda, M, N, K, tm, tn);
   return hipGetLastError();
 }
 
@@ -244,20 +301,18 @@ hipError_t launch_gemm_tile(double *c, const double *a, const double *b, int lda
 hipError_t launch_gemm_sub(double *c, const double *a, const double *b, int lda, int M, int N,
                            int K, hipStream_t s) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
-  const long long big = static_cast<long long>((M + 127) / 128) * ((N + 127) / 128);
-  if (big >= 512) return launch_gemm_tile<128, 128>(c, a, b, lda, M, N, K, s);
-  return launch_gemm_tile<64, 64>(c, a, b, lda, M, N, K, s);
+  if (gemm_uses_large_tiles(M, N)) return launch_gemm_tile<128, 128, 16>(c, a, b, lda, M, N, K, s);
+  return launch_gemm_tile<64, 64, 16>(c, a, b, lda, M, N, K, s);
 }
 
 hipError_t launch_swap_trsm(double *a, int lda, int pb, int w, int c0, int c1,
                             const int *swap_list, int nb, hipStream_t s) {
   if (c1 <= c0 || w <= 0) return hipSuccess;
-  constexpr int NS = 16;
-  const dim3 grid((c1 - c0 + NS - 1) / NS), block(256);
+  const dim3 grid((c1 - c0 + 15) / 16), block(256);
   switch (nb) {
-    case 64: hipLaunchKernelGGL((swap_trsm_kernel<64, NS>), grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
-    case 128: hipLaunchKernelGGL((swap_trsm_kernel<128, NS>), grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
-    case 256: hipLaunchKernelGGL((swap_trsm_kernel<256, NS>), grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
+    case 64: hipLaunchKernelGGL(swap_trsm_kernel<64>, grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
+    case 128: hipLaunchKernelGGL(swap_trsm_kernel<128>, grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
+    case 256: hipLaunchKernelGGL(swap_trsm_kernel<256>, grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

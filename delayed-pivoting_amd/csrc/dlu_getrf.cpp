@@ -63,8 +63,16 @@ struct Workspace {
   unsigned long long *d_po = nullptr;
   unsigned long long *d_phase = nullptr;
   unsigned seq = 0;
-  hipStream_t aux = nullptr;
-  hipEvent_t ev_ctl = nullptr, ev_aux = nullptr, ev_start = nullptr;
+  // Internal streams: panels run on `panel`, everything else on `update`.  When the device
+  // allows it the two streams own disjoint sets of CUs, so the panel kernel (which needs all of
+  // its workgroups resident at once) never waits for trailing-update workgroups to drain.
+  struct Streams {
+    int panel_cus = -1;  // CUs of the panel stream (0: both streams use the whole device)
+    hipStream_t panel = nullptr, update = nullptr;
+  };
+  Streams streams[4];
+  int nstreams = 0;
+  hipEvent_t ev_fork = nullptr, ev_ctl = nullptr, ev_next = nullptr, ev_join = nullptr;
   bool ready = false;
 };
 
@@ -94,10 +102,8 @@ hipError_t prepare(Workspace &w, int device, int steps) {
     if ((e = hipMalloc(&w.d_pv, kSearchPartials * sizeof(double))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_po, kSearchPartials * sizeof(unsigned long long))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_phase, 8 * sizeof(unsigned long long))) != hipSuccess) return e;
-    if ((e = hipStreamCreateWithFlags(&w.aux, hipStreamNonBlocking)) != hipSuccess) return e;
-    if ((e = hipEventCreateWithFlags(&w.ev_ctl, hipEventDisableTiming)) != hipSuccess) return e;
-    if ((e = hipEventCreateWithFlags(&w.ev_aux, hipEventDisableTiming)) != hipSuccess) return e;
-    if ((e = hipEventCreateWithFlags(&w.ev_start, hipEventDisableTiming)) != hipSuccess) return e;
+    for (hipEvent_t *ev : {&w.ev_fork, &w.ev_ctl, &w.ev_next, &w.ev_join})
+      if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
     w.ready = true;
   }
@@ -117,6 +123,44 @@ hipError_t prepare(Workspace &w, int device, int steps) {
   return hipSuccess;
 }
 
+// The stream pair whose panel stream owns `panel_cus` CUs (0: no partition).  CUs
+// [0, panel_cus) of the device mask go to the panel stream, the rest to the update stream.
+hipError_t stream_pair(Workspace &w, int panel_cus, Workspace::Streams *&out) {
+  for (int i = 0; i < w.nstreams; ++i)
+    if (w.streams[i].panel_cus == panel_cus) {
+      out = &w.streams[i];
+      return hipSuccess;
+    }
+  if (w.nstreams == 4) {
+    out = &w.streams[0];
+    return hipSuccess;
+  }
+  Workspace::Streams &st = w.streams[w.nstreams];
+  hipError_t e = hipSuccess;
+  bool masked = false;
+  if (panel_cus > 0 && panel_cus < w.cus) {
+    const int words = (w.cus + 31) / 32;
+    std::vector<uint32_t> pm(words, 0u), um(words, 0u);
+    for (int c = 0; c < w.cus; ++c) (c < panel_cus ? pm : um)[c / 32] |= 1u << (c % 32);
+    masked = hipExtStreamCreateWithCUMask(&st.panel, words, pm.data()) == hipSuccess &&
+             hipExtStreamCreateWithCUMask(&st.update, words, um.data()) == hipSuccess;
+    if (!masked) {
+      if (st.panel) hipStreamDestroy(st.panel);
+      if (st.update) hipStreamDestroy(st.update);
+      st.panel = st.update = nullptr;
+      (void)hipGetLastError();
+    }
+  }
+  if (!masked) {
+    if ((e = hipStreamCreateWithFlags(&st.panel, hipStreamNonBlocking)) != hipSuccess) return e;
+    if ((e = hipStreamCreateWithFlags(&st.update, hipStreamNonBlocking)) != hipSuccess) return e;
+  }
+  st.panel_cus = masked ? panel_cus : 0;
+  ++w.nstreams;
+  out = &st;
+  return hipSuccess;
+}
+
 inline double *at(double *a, int lda, int i, int j) {
   return a + static_cast<size_t>(i) + static_cast<size_t>(j) * static_cast<size_t>(lda);
 }
@@ -125,17 +169,20 @@ struct Run {
   Workspace &w;
   int m, n, lda, steps, nb;
   double *a;
-  hipStream_t s;
+  hipStream_t s;  // the caller's stream
   long long spin;
   bool profile = false;
   int panels = 0;
+  int max_groups = kMaxPanelGroups;  // workgroups the panel stream can hold at once
+  int forced_rpl = 0;                // rows per lane requested by the caller (0: automatic)
+  hipStream_t ps = nullptr, us = nullptr;  // internal panel / update streams
 
   // Opt-in launch timing: an event pair around each launch, on the launch's own stream.
   struct Timed {
     hipEvent_t begin, end;
     double flops;
   };
-  std::vector<Timed> t_update, t_panel, t_solve;
+  std::vector<Timed> t_update, t_update_big, t_panel, t_solve;
   hipError_t mark(std::vector<Timed> &v, hipStream_t st, double flops) {
     Timed t{nullptr, nullptr, flops};
     hipError_t e = hipEventCreate(&t.begin);
@@ -156,7 +203,7 @@ struct Run {
     return sec;
   }
   void release() {
-    for (auto *v : {&t_update, &t_panel, &t_solve})
+    for (auto *v : {&t_update, &t_update_big, &t_panel, &t_solve})
       for (Timed &t : *v) {
         if (t.begin) hipEventDestroy(t.begin);
         if (t.end) hipEventDestroy(t.end);
@@ -164,15 +211,18 @@ struct Run {
   }
 
   // Rows per lane of the panel kernel: the panel's rows must fit in the co-resident workgroups.
+  // The panel is held in registers: as few workgroups as possible (the most rows per lane the
+  // panel width allows), and no more than the panel stream can keep resident together.
   bool panel_shape(int pb, int &groups, int &rpl) const {
     const int rows = m - pb;
-    const int max_groups = std::min(kMaxPanelGroups, w.cus > 0 ? w.cus : kMaxPanelGroups);
-    const int rpl_max = nb == 256 ? 2 : 4;
-    for (rpl = 1; rpl <= rpl_max; rpl *= 2) {
-      groups = (rows + 64 * rpl - 1) / (64 * rpl);
-      if (groups <= max_groups) return true;
+    if (forced_rpl > 0) {
+      rpl = forced_rpl;
+    } else {
+      rpl = nb == 256 ? 2 : 4;
+      if (nb == 256 && (rows + 63) / 64 <= max_groups) rpl = 1;  // avoid the spilling variant
     }
-    return false;
+    groups = (rows + 64 * rpl - 1) / (64 * rpl);
+    return groups <= max_groups;
   }
 
   hipError_t panel(int pb, int pe, int positioned, int parity) {
@@ -196,15 +246,15 @@ struct Run {
     p.swap_list = w.d_lists + parity * kListStride;
     p.spin_limit = spin;
     p.phase_ticks = profile ? w.d_phase : nullptr;
-    hipError_t e = profile ? mark(t_panel, s, 0.0) : hipSuccess;
-    if (e == hipSuccess) e = launch_panel(p, nb, rpl, s);
-    if (e == hipSuccess && profile) e = close(t_panel, s);
+    hipError_t e = profile ? mark(t_panel, ps, 0.0) : hipSuccess;
+    if (e == hipSuccess) e = launch_panel(p, nb, rpl, ps);
+    if (e == hipSuccess && profile) e = close(t_panel, ps);
     if (e != hipSuccess) return e;
     ++panels;
-    if ((e = hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost, s)) !=
+    if ((e = hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost, ps)) !=
         hipSuccess)
       return e;
-    return hipEventRecord(w.ev_ctl, s);
+    return hipEventRecord(w.ev_ctl, ps);
   }
 
   // Interchange + U block row + trailing update of columns [c0, c1) for the panel rows
@@ -218,17 +268,23 @@ struct Run {
     const int r0 = pb + wdt;
     if (m - r0 <= 0) return hipSuccess;
     const double flops = 2.0 * (m - r0) * static_cast<double>(c1 - c0) * wdt;
-    if (profile && (e = mark(t_update, st, flops)) != hipSuccess) return e;
+    std::vector<Timed> &tv = gemm_uses_large_tiles(m - r0, c1 - c0) ? t_update_big : t_update;
+    if (profile && (e = mark(tv, st, flops)) != hipSuccess) return e;
     e = launch_gemm_sub(at(a, lda, r0, c0), at(a, lda, r0, pb), at(a, lda, pb, c0), lda, m - r0,
                         c1 - c0, wdt, st);
-    if (e == hipSuccess && profile) e = close(t_update, st);
+    if (e == hipSuccess && profile) e = close(tv, st);
     return e;
   }
 
+  // Runs on the panel stream `ps` and the update stream `us`, forked from and joined back to the
+  // caller's stream.  Per panel the host reads the control block once to learn where the panel
+  // stopped; every pivot decision inside the panel is taken on the device.
   hipError_t factor(int *info, bool lookahead) {
     hipError_t e;
+    if ((e = hipEventRecord(w.ev_fork, s)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(ps, w.ev_fork, 0)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(us, w.ev_fork, 0)) != hipSuccess) return e;
     int pb = 0, parity = 0;
-    bool aux_pending = false;
     int pe = std::min(pb + nb, steps);
     if ((e = panel(pb, pe, 0, parity)) != hipSuccess) return e;
     for (;;) {
@@ -242,61 +298,57 @@ struct Run {
       const int stop = c.stop;
       const int wdt = stop - pb;
       const int *list = w.d_lists + parity * kListStride;
+      if ((e = hipStreamWaitEvent(us, w.ev_ctl, 0)) != hipSuccess) return e;
       if (!c.zero_column) {
         const int pe2 = std::min(pe + nb, steps);
         if (pe < steps && lookahead) {
-          // Next panel's columns on the caller's stream, far columns on the auxiliary stream.
-          if (aux_pending && (e = hipStreamWaitEvent(s, w.ev_aux, 0)) != hipSuccess) return e;
-          if ((e = update(pb, wdt, pe, pe2, list, s)) != hipSuccess) return e;
-          if ((e = hipStreamWaitEvent(w.aux, w.ev_ctl, 0)) != hipSuccess) return e;
-          if ((e = update(pb, wdt, pe2, n, list, w.aux)) != hipSuccess) return e;
-          if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, w.aux)) != hipSuccess) return e;
-          if ((e = hipEventRecord(w.ev_aux, w.aux)) != hipSuccess) return e;
-          aux_pending = true;
-          pb = pe;
-          pe = pe2;
-          parity ^= 1;
-          if ((e = panel(pb, pe, 0, parity)) != hipSuccess) return e;
-          continue;
-        }
-        if (aux_pending) {
-          if ((e = hipStreamWaitEvent(s, w.ev_aux, 0)) != hipSuccess) return e;
-          aux_pending = false;
-        }
-        if ((e = update(pb, wdt, pe, n, list, s)) != hipSuccess) return e;
-        if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, s)) != hipSuccess) return e;
-        if (pe >= steps) {
-          *info = steps;
-          break;
+          // The next panel's columns first, then the far columns while the next panel runs.
+          if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
+          if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
+          if ((e = update(pb, wdt, pe2, n, list, us)) != hipSuccess) return e;
+          if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
+        } else {
+          if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
+          if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
+          if (pe >= steps) {
+            *info = steps;
+            break;
+          }
+          if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
         }
         pb = pe;
         pe = pe2;
-        parity ^= 1;
-        if ((e = panel(pb, pe, 0, parity)) != hipSuccess) return e;
-        continue;
+      } else {
+        // Zero column at `stop`: bring the far columns up to date with the accepted prefix
+        // (delayed_dgetrf.cpp:183-186), search the whole active block, interchange, restart.
+        if (wdt > 0) {
+          if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
+          if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
+        }
+        if ((e = launch_active_search(a, lda, m, n, stop, w.d_pv, w.d_po, w.d_ctl, us)) !=
+            hipSuccess)
+          return e;
+        if ((e = launch_fallback_apply(a, lda, m, n, stop, w.d_ipiv, w.d_jpiv, w.d_ctl, us)) !=
+            hipSuccess)
+          return e;
+        if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
+        pb = stop;
+        pe = std::min(pb + nb, steps);
       }
-      // Zero column at `stop`: bring the far columns up to date with the accepted prefix
-      // (delayed_dgetrf.cpp:183-186), search the whole active block, interchange, restart.
-      if (aux_pending) {
-        if ((e = hipStreamWaitEvent(s, w.ev_aux, 0)) != hipSuccess) return e;
-        aux_pending = false;
-      }
-      if (wdt > 0) {
-        if ((e = update(pb, wdt, pe, n, list, s)) != hipSuccess) return e;
-        if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, s)) != hipSuccess) return e;
-      }
-      if ((e = launch_active_search(a, lda, m, n, stop, w.d_pv, w.d_po, w.d_ctl, s)) != hipSuccess)
-        return e;
-      if ((e = launch_fallback_apply(a, lda, m, n, stop, w.d_ipiv, w.d_jpiv, w.d_ctl, s)) !=
-          hipSuccess)
-        return e;
-      pb = stop;
-      pe = std::min(pb + nb, steps);
       parity ^= 1;
-      if ((e = panel(pb, pe, 1, parity)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(ps, w.ev_next, 0)) != hipSuccess) return e;
+      if ((e = panel(pb, pe, c.zero_column ? 1 : 0, parity)) != hipSuccess) return e;
     }
-    if (aux_pending && (e = hipStreamWaitEvent(s, w.ev_aux, 0)) != hipSuccess) return e;
     return hipSuccess;
+  }
+
+  // The caller's stream resumes after everything queued on the internal streams.
+  hipError_t join() {
+    hipError_t e = hipEventRecord(w.ev_join, ps);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, w.ev_join, 0);
+    if (e == hipSuccess) e = hipEventRecord(w.ev_join, us);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, w.ev_join, 0);
+    return e;
   }
 };
 
@@ -313,7 +365,7 @@ dlu_status getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv
   if (info == nullptr) return DLU_INVALID_ARGUMENT;
   *info = -1;
   if (!valid_arguments(m, n, d_a, lda, ipiv, jpiv)) return DLU_INVALID_ARGUMENT;
-  int nb = (opt && opt->nb) ? opt->nb : 256;
+  int nb = (opt && opt->nb) ? opt->nb : 128;
   if (nb != 64 && nb != 128 && nb != 256) return DLU_INVALID_ARGUMENT;
   const bool lookahead = !(opt && opt->lookahead < 0);
   const long long spin =
@@ -345,6 +397,33 @@ dlu_status getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv
   }
   Run run{w, m, n, lda, steps, nb, d_a, stream, spin};
   run.profile = opt && opt->profile;
+  if (opt && opt->rows_per_lane) {
+    if (opt->rows_per_lane != 1 && opt->rows_per_lane != 2 && opt->rows_per_lane != 4) {
+      *info = -1;
+      return DLU_INVALID_ARGUMENT;
+    }
+    run.forced_rpl = opt->rows_per_lane;
+  }
+  {
+    // CUs for the panel stream: enough for one resident workgroup per panel row block, at
+    // least 16 and at most half the device (opt->panel_cus > 0 forces a size, < 0 disables).
+    int groups = 0, rpl = 0;
+    run.max_groups = std::min(kMaxPanelGroups, w.cus);
+    run.panel_shape(0, groups, rpl);
+    int want = ((groups + 15) / 16) * 16;
+    want = std::max(16, std::min(want, w.cus / 2));
+    if (opt && opt->panel_cus > 0) want = std::min(opt->panel_cus, w.cus);
+    if (opt && opt->panel_cus < 0) want = 0;
+    if (groups > want && want > 0) want = 0;  // too tall for a partition: share the device
+    Workspace::Streams *st = nullptr;
+    if ((e = stream_pair(w, want, st)) != hipSuccess) {
+      *info = -1;
+      return to_status(e);
+    }
+    run.ps = st->panel;
+    run.us = st->update;
+    if (st->panel_cus > 0) run.max_groups = std::min(kMaxPanelGroups, st->panel_cus);
+  }
   if (run.profile && (e = hipMemsetAsync(w.d_phase, 0, 8 * sizeof(unsigned long long), stream)) !=
                          hipSuccess) {
     *info = -1;
@@ -367,8 +446,8 @@ dlu_status getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv
     e = run.factor(&out_info, lookahead);
     if (e == hipSuccess) *info = out_info;
   }
-  // Whatever happened, leave no work of this call queued on the auxiliary stream.
-  hipError_t e2 = hipStreamSynchronize(w.aux);
+  // Whatever happened, leave no work of this call queued on the internal streams.
+  hipError_t e2 = run.join();
   if (e == hipSuccess) e = e2;
   if (e == hipSuccess)
     e = hipMemcpyAsync(ipiv, w.d_ipiv, steps * sizeof(int), hipMemcpyDeviceToHost, stream);
@@ -391,8 +470,12 @@ dlu_status getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv
     rep->exact_ties = c.exact_ties;
     rep->min_margin = c.min_margin;
     if (run.profile) {
-      rep->update_launches = static_cast<int>(run.t_update.size());
-      rep->update_seconds = Run::total(run.t_update, &rep->update_flops);
+      rep->large_update_launches = static_cast<int>(run.t_update_big.size());
+      rep->large_update_seconds =
+          Run::total(run.t_update_big, &rep->large_update_flops);
+      rep->update_launches = static_cast<int>(run.t_update.size()) + rep->large_update_launches;
+      rep->update_seconds = Run::total(run.t_update, &rep->update_flops) + rep->large_update_seconds;
+      rep->update_flops += rep->large_update_flops;
       rep->panel_launches = static_cast<int>(run.t_panel.size());
       rep->panel_seconds = Run::total(run.t_panel, nullptr);
       rep->solve_launches = static_cast<int>(run.t_solve.size());

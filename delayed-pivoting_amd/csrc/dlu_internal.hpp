@@ -70,6 +70,10 @@ hipError_t launch_swap_trsm(double *a, int lda, int pb, int w, int c0, int c1,
                             const int *swap_list, int nb, hipStream_t s);
 
 // C(M x N) -= A(M x K) * B(K x N), column-major, all with leading dimension lda.
+// Large problems use 128 x 128 tiles, small ones 64 x 64 (gemm_uses_large_tiles tells which).
+inline bool gemm_uses_large_tiles(int M, int N) {
+  return static_cast<long long>((M + 127) / 128) * ((N + 127) / 128) >= 512;
+}
 hipError_t launch_gemm_sub(double *c, const double *a, const double *b, int lda, int M, int N,
                            int K, hipStream_t s);
 

@@ -80,23 +80,62 @@ __device__ __forceinline__ bool beats(double v, int row, double bv, int brow) {
   return v > bv || (v == bv && row < brow);
 }
 
-// Wave-wide (value, row, second value, tag) reduction keeping the two largest magnitudes.
-__device__ __forceinline__ void wave_top2(double &v1, int &row, double &v2, int &tag) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double ov1 = __shfl_xor(v1, off);
-    const int orow = __shfl_xor(row, off);
-    const double ov2 = __shfl_xor(v2, off);
-    const int otag = __shfl_xor(tag, off);
-    if (beats(ov1, orow, v1, row)) {
-      v2 = fmax(v1, ov2);
-      v1 = ov1;
-      row = orow;
-      tag = otag;
-    } else {
-      v2 = fmax(v2, ov1);
-    }
+__device__ __forceinline__ unsigned lo32(double x) {
+  return static_cast<unsigned>(static_cast<unsigned long long>(__double_as_longlong(x)));
+}
+__device__ __forceinline__ unsigned hi32(double x) {
+  return static_cast<unsigned>(static_cast<unsigned long long>(__double_as_longlong(x)) >> 32);
+}
+__device__ __forceinline__ double mk(unsigned lo, unsigned hi) {
+  return __longlong_as_double(
+      static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp(unsigned x) {
+  return __builtin_amdgcn_update_dpp(0u, x, CTRL, 0xF, 0xF, false);
+}
+
+// Wave-wide max / min: four DPP steps inside each row of 16 lanes, then the gfx950 permlane
+// swaps across rows (with both operands equal, the pair returned holds the value of this lane
+// and of its partner 16 or 32 lanes away).
+__device__ __forceinline__ double wave_max(double x) {
+#define DLU_DPP_MAX(CTRL) x = fmax(x, mk(dpp<CTRL>(lo32(x)), dpp<CTRL>(hi32(x))));
+  DLU_DPP_MAX(0xB1)   // quad_perm [1,0,3,2]
+  DLU_DPP_MAX(0x4E)   // quad_perm [2,3,0,1]
+  DLU_DPP_MAX(0x141)  // row_half_mirror
+  DLU_DPP_MAX(0x140)  // row_mirror
+#undef DLU_DPP_MAX
+  {
+    const auto a = __builtin_amdgcn_permlane16_swap(lo32(x), lo32(x), false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(hi32(x), hi32(x), false, false);
+    x = fmax(mk(a[0], b[0]), mk(a[1], b[1]));
   }
+  {
+    const auto a = __builtin_amdgcn_permlane32_swap(lo32(x), lo32(x), false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi32(x), hi32(x), false, false);
+    x = fmax(mk(a[0], b[0]), mk(a[1], b[1]));
+  }
+  return x;
+}
+
+__device__ __forceinline__ int wave_min(int x) {
+  unsigned u = static_cast<unsigned>(x);
+#define DLU_DPP_MIN(CTRL) u = static_cast<unsigned>(min(static_cast<int>(u), static_cast<int>(dpp<CTRL>(u))));
+  DLU_DPP_MIN(0xB1)
+  DLU_DPP_MIN(0x4E)
+  DLU_DPP_MIN(0x141)
+  DLU_DPP_MIN(0x140)
+#undef DLU_DPP_MIN
+  {
+    const auto a = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    u = static_cast<unsigned>(min(static_cast<int>(a[0]), static_cast<int>(a[1])));
+  }
+  {
+    const auto a = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    u = static_cast<unsigned>(min(static_cast<int>(a[0]), static_cast<int>(a[1])));
+  }
+  return static_cast<int>(u);
 }
 
 // Sixteen columns of one row, kept as a register vector so that a wave-uniform runtime column
@@ -113,6 +152,7 @@ template <int NB, int RPL>
 __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
   constexpr int R = 64 * RPL;
   constexpr int SW = slot_words(NB);
+  constexpr int NW = NB / 16;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = blockIdx.x;
@@ -121,14 +161,13 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
   const int col0 = wave * 16;
   const int m = P.m;
 
-  __shared__ double s_lbuf[R];
-  __shared__ double s_urow[NB];
-  __shared__ double s_krow[NB];
+  __shared__ double s_col[2][R];   // the column being pivoted, for every wave's argmax
+  __shared__ double s_u[NW][16];   // per-wave slice of the pivot row
+  __shared__ double s_k[NW][16];   // per-wave slice of the old row k (for the owner of row p)
+  __shared__ double s_pub[NW][2][16];  // per-wave slices of the candidate row and of row k
   __shared__ double s_cv[kMaxPanelGroups];
   __shared__ double s_cv2[kMaxPanelGroups];
   __shared__ int s_crow[kMaxPanelGroups];
-  __shared__ double s_loc_v, s_loc_v2;
-  __shared__ int s_loc_row;
   __shared__ int s_fail;
   __shared__ int s_hist[NB];
   __shared__ int s_cur[NB];
@@ -146,6 +185,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
                     ? P.a[static_cast<size_t>(row) + static_cast<size_t>(P.pb + col) * P.lda]
                     : 0.0;
     }
+    if (wave == 0) s_col[0][64 * r + lane] = v[r][0];
   }
   if (threadIdx.x == 0) s_fail = 0;
   const bool deferred = P.ctl->deferred != 0;  // a speculative restart after a deferral
@@ -168,77 +208,86 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
   const bool timed = P.phase_ticks != nullptr && g == 0 && threadIdx.x == 0;
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   unsigned long long tp = timed ? wall_clock64() : 0;
-#define DLU_PHASE(i)                         \
-  if (timed) {                               \
+#define DLU_PHASE(i)                              \
+  if (timed) {                                    \
     const unsigned long long tn = wall_clock64(); \
-    ph[i] += tn - tp;                        \
-    tp = tn;                                 \
+    ph[i] += tn - tp;                             \
+    tp = tn;                                      \
   }
 
   for (int c = 0; c < width; ++c) {
     const int k = P.pb + c;
     const unsigned seq = P.seq_base + static_cast<unsigned>(c) + 1u;
     const int par = static_cast<int>(seq & 1u);
+    const int buf = c & 1;
     const bool forced = P.positioned && c == 0;
-    const int owner_wave = c >> 4;
     const int cc = c & 15;
+    const int owner_wave = c >> 4;
 
-    // Local candidate of this workgroup (only the wave holding column c has it).
-    if (wave == owner_wave) {
-      double b1 = -1.0, b2 = -1.0;
-      int brow = INT_MAX, tag = 0;
+    // Local candidate: every wave reduces this workgroup's part of column c from LDS.  The
+    // winner is the largest magnitude, the lowest row among equal magnitudes (NaN never wins:
+    // max ignores it and it equals nothing), and v2 the largest magnitude of the other rows.
+    double xc[RPL];  // this workgroup's part of column c (kept for the multipliers)
+    double ax[RPL];
+    bool ok[RPL];
 #pragma unroll
-      for (int r = 0; r < RPL; ++r) {
-        const int row = base + 64 * r + lane;
-        const bool ok = row >= k && row < m && (!forced || row == k);
-        if (ok) {
-          const double x = fabs(v[r][cc]);
-          if (beats(x, row, b1, brow)) {
-            b2 = fmax(b1, b2);
-            b1 = x;
-            brow = row;
-          } else {
-            b2 = fmax(b2, x);
-          }
-        }
-      }
-      wave_top2(b1, brow, b2, tag);
-      if (lane == 0) {
-        s_loc_v = b1;
-        s_loc_v2 = b2;
-        s_loc_row = brow;
-      }
-    }
-    __syncthreads();
-    DLU_PHASE(0)
-    const int lrow = s_loc_row;
-
-    // Publish: header, candidate row, and row k when this workgroup owns it.
-    uint64_t *slot = P.slots + (static_cast<size_t>(par) * P.groups + g) * SW;
-    uint64_t *rk = P.rowk + static_cast<size_t>(par) * 2 * NB;
-    if (threadIdx.x == 0) {
-      ll_put_double(slot + 0, seq, s_loc_v);
-      ll_put_double(slot + 2, seq, s_loc_v2);
-      ll_put(slot + 4, seq, static_cast<unsigned>(lrow));
-    }
+    for (int r = 0; r < RPL; ++r) xc[r] = s_col[buf][64 * r + lane];
+    double m1 = -1.0;
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
       const int row = base + 64 * r + lane;
-      uint64_t *dst = row == lrow ? slot + kSlotHeaderWords : (row == k ? rk : nullptr);
-      if (dst != nullptr) {
-        // One lane per wave publishes; keep the loop rolled (runtime j indexes the vector).
-#pragma unroll 1
-        for (int j = 0; j < 16; ++j)
-          if (col0 + j < width) ll_put_double(dst + 2 * (col0 + j), seq, v[r][j]);
-        if (row == lrow && row == k) {
-#pragma unroll 1
-          for (int j = 0; j < 16; ++j)
-            if (col0 + j < width) ll_put_double(rk + 2 * (col0 + j), seq, v[r][j]);
+      ok[r] = row >= k && row < m && (!forced || row == k);
+      ax[r] = ok[r] ? fabs(xc[r]) : -1.0;
+      m1 = fmax(m1, ax[r]);
+    }
+    const double lv1 = wave_max(m1);
+    int myrow = INT_MAX;
+#pragma unroll
+    for (int r = RPL - 1; r >= 0; --r)
+      myrow = (ok[r] && ax[r] == lv1) ? base + 64 * r + lane : myrow;
+    const int lrow = wave_min(myrow);
+    double m2 = -1.0;
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) m2 = fmax(m2, (base + 64 * r + lane != lrow) ? ax[r] : -1.0);
+    const double lv2 = wave_max(m2);
+    DLU_PHASE(0)
+
+    // Publish: header (thread 0), this wave's 16 columns of the candidate row and of row k.
+    uint64_t *slot = P.slots + (static_cast<size_t>(par) * P.groups + g) * SW;
+    uint64_t *rk = P.rowk + static_cast<size_t>(par) * 2 * NB;
+    if (threadIdx.x == 0) {
+      ll_put_double(slot + 0, seq, lv1);
+      ll_put_double(slot + 2, seq, lv2);
+      ll_put(slot + 4, seq, static_cast<unsigned>(lrow));
+    }
+    {
+      // Stage the two rows in LDS so 32 lanes each store one exchange word (one store
+      // instruction per wave instead of 32 from a single lane).
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) {
+        const int row = base + 64 * r + lane;
+        if (row == lrow) {
+#pragma unroll
+          for (int j = 0; j < 16; ++j) s_pub[wave][0][j] = v[r][j];
+        }
+        if (row == k) {
+#pragma unroll
+          for (int j = 0; j < 16; ++j) s_pub[wave][1][j] = v[r][j];
         }
       }
+      wave_sync_lds();
+      const int j = (lane & 31) >> 1, half = lane & 1;
+      const bool cand_here = lrow >= base && lrow < base + R;
+      const bool k_here = k >= base && k < base + R;
+      if (col0 + j < width && (lane < 32 ? cand_here : k_here)) {
+        const double x = s_pub[wave][lane >> 5][j];
+        const unsigned pay = half ? hi32(x) : lo32(x);
+        uint64_t *dstw = (lane < 32 ? slot + kSlotHeaderWords : rk) + 2 * (col0 + j) + half;
+        ll_put(dstw, seq, pay);
+      }
     }
-
     DLU_PHASE(1)
+
     // Gather every workgroup's header.
     if (threadIdx.x < P.groups) {
       const uint64_t *hs = P.slots + (static_cast<size_t>(par) * P.groups + threadIdx.x) * SW;
@@ -253,72 +302,105 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
     if (s_fail) goto leave;
 
     {
-      // Every wave reduces all candidates itself (no further barrier needed for the winner).
-      double g1 = -1.0, g2 = -1.0;
-      int grow = INT_MAX, gwin = 0;
-      for (int i = lane; i < P.groups; i += 64) {
-        const double cv = s_cv[i];
-        const int crow = s_crow[i];
-        if (beats(cv, crow, g1, grow)) {
-          g2 = fmax(fmax(g1, g2), s_cv2[i]);
-          g1 = cv;
-          grow = crow;
-          gwin = i;
-        } else {
-          g2 = fmax(fmax(g2, cv), s_cv2[i]);
-        }
+      // Every wave reduces all candidates itself, with the same rule.
+      constexpr int GQ = kMaxPanelGroups / 64;
+      double gv[GQ], gw[GQ];
+      int gq[GQ];
+#pragma unroll
+      for (int q = 0; q < GQ; ++q) {  // unconditional LDS reads, then selects
+        gv[q] = s_cv[lane + 64 * q];
+        gw[q] = s_cv2[lane + 64 * q];
+        gq[q] = s_crow[lane + 64 * q];
       }
-      wave_top2(g1, grow, g2, gwin);
-      DLU_PHASE(3)
-
+      double gm = -1.0;
+#pragma unroll
+      for (int q = 0; q < GQ; ++q) {
+        gv[q] = lane + 64 * q < P.groups ? gv[q] : -1.0;
+        gm = fmax(gm, gv[q]);
+      }
+      const double g1 = wave_max(gm);
       if (!(g1 > 0.0)) {  // zero (or NaN-only) column: stop the panel at k
         stop = k;
         zero_col = 1;
         break;
       }
-      const int p = grow;
+      int gr = INT_MAX;
+#pragma unroll
+      for (int q = 0; q < GQ; ++q) gr = (gv[q] == g1) ? min(gr, gq[q]) : gr;
+      const int p = wave_min(gr);
+      const int gwin = (p - P.pb) / R;  // rows are dealt to workgroups in blocks of R
+      double gs = -1.0;
+#pragma unroll
+      for (int q = 0; q < GQ; ++q) gs = fmax(gs, (lane + 64 * q == gwin) ? gw[q] : gv[q]);
+      const double g2 = wave_max(gs);
+      DLU_PHASE(3)
+      const bool p_here = p != k && p >= base && p < base + R;
 
-      // This wave's 16 columns of the pivot row and of row k.
-      if (lane < 16 && col0 + lane < width) {
-        const uint64_t *ws = P.slots + (static_cast<size_t>(par) * P.groups + gwin) * SW +
-                             kSlotHeaderWords + 2 * (col0 + lane);
+      // One exchange round trip for everything this wave needs: its 16 columns of the pivot row
+      // (lanes 0-15), the pivot itself (lane 16), and, when row p lives here, the old row k's
+      // 16 columns (lanes 32-47) and its column-c value (lane 17).
+      const uint64_t *wrow =
+          P.slots + (static_cast<size_t>(par) * P.groups + gwin) * SW + kSlotHeaderWords;
+      const uint64_t *src = nullptr;
+      if (lane < 16 && col0 + lane < width) src = wrow + 2 * (col0 + lane);
+      if (lane == 16) src = wrow + 2 * c;
+      if (p_here && lane == 17) src = rk + 2 * c;
+      if (p_here && lane >= 32 && lane < 48 && col0 + lane - 32 < width)
+        src = rk + 2 * (col0 + lane - 32);
+      double got = 0.0;
+      if (src != nullptr) {
         unsigned w[2];
-        if (!ll_get<2>(ws, seq, w, P, t0)) s_fail = 1;
-        s_urow[col0 + lane] = join_double(w[0], w[1]);
-        if (p != k) {
-          if (!ll_get<2>(rk + 2 * (col0 + lane), seq, w, P, t0)) s_fail = 1;
-          s_krow[col0 + lane] = join_double(w[0], w[1]);
-        }
+        if (!ll_get<2>(src, seq, w, P, t0)) s_fail = 1;
+        got = join_double(w[0], w[1]);
       }
+      if (lane < 16) s_u[wave][lane] = got;
+      if (lane >= 32 && lane < 48) s_k[wave][lane - 32] = got;
+      const double piv = __shfl(got, 16);
+      const double kc = __shfl(got, 17);
       wave_sync_lds();
       DLU_PHASE(4)
 
-      // Interchange rows k and p on this workgroup's rows.
+      // Interchange rows k and p (only the lanes holding them), form the multipliers (column c
+      // times the reciprocal of the pivot) and apply the rank-1 update to the panel columns
+      // right of c.  Rows at or above k keep their values bit for bit.
+      const double recip = 1.0 / piv;
+      double l[RPL];
+      bool below[RPL];
 #pragma unroll
       for (int r = 0; r < RPL; ++r) {
         const int row = base + 64 * r + lane;
-        if (row == k) {
-#pragma unroll 1
-          for (int j = 0; j < 16; ++j) v[r][j] = s_urow[col0 + j];
-        } else if (row == p) {
-#pragma unroll 1
-          for (int j = 0; j < 16; ++j) v[r][j] = s_krow[col0 + j];
+        if (row == k) {  // divergent: only the lane holding row k
+#pragma unroll
+          for (int j = 0; j < 16; ++j) v[r][j] = s_u[wave][j];
+        } else if (p_here && row == p) {  // only the lane holding row p
+#pragma unroll
+          for (int j = 0; j < 16; ++j) v[r][j] = s_k[wave][j];
+        }
+        below[r] = row > k && row < m;
+        l[r] = ((p_here && row == p) ? kc : xc[r]) * recip;
+      }
+      const int jc = c - col0;  // this wave's column index of c (may be out of 0..15)
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if (j == jc) {  // wave-uniform: the owner stores the multipliers
+#pragma unroll
+          for (int r = 0; r < RPL; ++r)
+            if (below[r]) v[r][j] = l[r];
+        } else if (j > jc && col0 + j < width) {  // wave-uniform column skip
+          const double u = s_u[wave][j];
+#pragma unroll
+          for (int r = 0; r < RPL; ++r)
+            if (below[r]) v[r][j] = fma(-l[r], u, v[r][j]);
         }
       }
-
-      // Multipliers: column c times the reciprocal of the pivot.
-      if (wave == owner_wave) {
-        const double recip = 1.0 / s_urow[c];
+      // The owner of column c+1 hands its updated column to every wave.
+      if (jc + 1 >= 0 && jc + 1 < 16 && c + 1 < width) {
 #pragma unroll
-        for (int r = 0; r < RPL; ++r) {
-          const int row = base + 64 * r + lane;
-          double l = 0.0;
-          if (row > k && row < m) {
-            l = v[r][cc] * recip;
-            v[r][cc] = l;
+        for (int j = 0; j < 16; ++j)
+          if (j == jc + 1) {
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) s_col[buf ^ 1][64 * r + lane] = v[r][j];
           }
-          s_lbuf[64 * r + lane] = l;
-        }
       }
       if (g == 0 && threadIdx.x == 0) {
         if (!forced) P.ipiv[k] = p;  // a forced step keeps the fallback's recorded row
@@ -332,22 +414,10 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
         }
         if (P.margins) P.margins[k] = margin;
       }
-      __syncthreads();
       DLU_PHASE(5)
-      if (s_fail) goto leave;
-
-      // Rank-1 update of this workgroup's rows, panel columns right of c.
-#pragma unroll
-      for (int r = 0; r < RPL; ++r) {
-        const int row = base + 64 * r + lane;
-        if (row > k && row < m) {
-          const double l = s_lbuf[64 * r + lane];
-#pragma unroll
-          for (int j = 0; j < 16; ++j)
-            if (col0 + j > c && col0 + j < width) v[r][j] = fma(-l, s_urow[col0 + j], v[r][j]);
-        }
-      }
+      __syncthreads();
       DLU_PHASE(6)
+      if (s_fail) goto leave;
     }
   }
 

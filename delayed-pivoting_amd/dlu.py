@@ -29,7 +29,8 @@ class Status(enum.IntEnum):
 
 class Options(ctypes.Structure):
     _fields_ = [("nb", ctypes.c_int), ("lookahead", ctypes.c_int),
-                ("spin_limit_ticks", ctypes.c_longlong), ("profile", ctypes.c_int)]
+                ("spin_limit_ticks", ctypes.c_longlong), ("profile", ctypes.c_int),
+                ("panel_cus", ctypes.c_int), ("rows_per_lane", ctypes.c_int)]
 
 
 class Report(ctypes.Structure):
@@ -37,7 +38,9 @@ class Report(ctypes.Structure):
                 ("near_ties", ctypes.c_int), ("exact_ties", ctypes.c_int),
                 ("min_margin", ctypes.c_double),
                 ("update_launches", ctypes.c_int), ("update_seconds", ctypes.c_double),
-                ("update_flops", ctypes.c_double), ("panel_launches", ctypes.c_int),
+                ("update_flops", ctypes.c_double), ("large_update_launches", ctypes.c_int),
+                ("large_update_seconds", ctypes.c_double), ("large_update_flops", ctypes.c_double),
+                ("panel_launches", ctypes.c_int),
                 ("panel_seconds", ctypes.c_double), ("solve_launches", ctypes.c_int),
                 ("solve_seconds", ctypes.c_double),
                 ("panel_phase_seconds", ctypes.c_double * 8)]

@@ -66,10 +66,13 @@ const char *dlu_status_string(dlu_status status);
 
 /* Tuning knobs.  Zero-initialised fields take the defaults. */
 typedef struct dlu_options {
-  int nb;          /* panel width: 64, 128 or 256 (default 256) */
+  int nb;          /* panel width: 64, 128 or 256 (default 128) */
   int lookahead;   /* 1 (default): factor panel j+1 while panel j's far update runs; -1: off */
   long long spin_limit_ticks; /* bound on any in-kernel wait, 100 MHz ticks (default 2e8 = 2 s) */
   int profile;     /* nonzero: time every panel / solve / update launch with HIP events */
+  int panel_cus;   /* CUs reserved for panel kernels while updates run on the rest:
+                      0 = automatic, > 0 = that many, < 0 = no reservation */
+  int rows_per_lane; /* panel rows held per lane: 0 = automatic, else 1, 2 or 4 */
 } dlu_options;
 
 /* What happened during one factorization. */
@@ -84,6 +87,9 @@ typedef struct dlu_report {
   int update_launches;
   double update_seconds;
   double update_flops;
+  int large_update_launches;   /* the subset that ran the 128 x 128-tile kernel */
+  double large_update_seconds;
+  double large_update_flops;
   int panel_launches;
   double panel_seconds;
   int solve_launches;
