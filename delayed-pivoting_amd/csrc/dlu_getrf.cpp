@@ -80,6 +80,12 @@ constexpr int kMaxDevices = 64;
 constexpr int kSearchPartials = 2048;
 constexpr int kListStride = 4 * 256 + 4;  // swap_list_ints(256), enough for every nb
 
+// Exchange buffers, sized for the largest panel width (256) and workgroup count.
+constexpr size_t slot_bytes() {
+  return 2ull * kMaxPanelGroups * static_cast<size_t>(slot_words(256)) * sizeof(uint64_t);
+}
+constexpr size_t rowk_bytes() { return 2ull * 2 * 256 * sizeof(uint64_t); }
+
 std::mutex g_lock;
 Workspace g_ws[kMaxDevices];
 
@@ -90,14 +96,12 @@ hipError_t prepare(Workspace &w, int device, int steps) {
     hipError_t e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) return e;
     w.cus = prop.multiProcessorCount;
-    const size_t slot_bytes =
-        2ull * kMaxPanelGroups * static_cast<size_t>(slot_words(256)) * sizeof(uint64_t);
     if ((e = hipMalloc(&w.d_ctl, sizeof(Control))) != hipSuccess) return e;
     if ((e = hipHostMalloc(&w.h_ctl, sizeof(Control))) != hipSuccess) return e;
-    if ((e = hipMalloc(&w.d_slots, slot_bytes)) != hipSuccess) return e;
-    if ((e = hipMemset(w.d_slots, 0, slot_bytes)) != hipSuccess) return e;
-    if ((e = hipMalloc(&w.d_rowk, 2 * 2 * 256 * sizeof(uint64_t))) != hipSuccess) return e;
-    if ((e = hipMemset(w.d_rowk, 0, 2 * 2 * 256 * sizeof(uint64_t))) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_slots, slot_bytes())) != hipSuccess) return e;
+    if ((e = hipMemset(w.d_slots, 0, slot_bytes())) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_rowk, rowk_bytes())) != hipSuccess) return e;
+    if ((e = hipMemset(w.d_rowk, 0, rowk_bytes())) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_lists, 2 * kListStride * sizeof(int))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_pv, kSearchPartials * sizeof(double))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_po, kSearchPartials * sizeof(unsigned long long))) != hipSuccess) return e;
@@ -236,6 +240,14 @@ struct Run {
     p.pe = pe;
     p.positioned = positioned;
     p.groups = groups;
+    // Step tags are 16 bits wide in the exchange headers: before they would wrap, every
+    // exchange word is cleared (tag 0, never a valid step) and numbering restarts at 1.
+    if (w.seq + static_cast<unsigned>(nb) + 1u > kMaxStepTag) {
+      hipError_t e = hipMemsetAsync(w.d_slots, 0, slot_bytes(), ps);
+      if (e == hipSuccess) e = hipMemsetAsync(w.d_rowk, 0, rowk_bytes(), ps);
+      if (e != hipSuccess) return e;
+      w.seq = 0;
+    }
     p.seq_base = w.seq;
     w.seq += static_cast<unsigned>(nb) + 1u;
     p.ipiv = w.d_ipiv;

@@ -30,10 +30,14 @@ struct Control {
   unsigned abort_flag;  // set by a panel workgroup whose wait expired; the others leave too
 };
 
-// Exchange buffer geometry of the panel kernel (in 64-bit words).  Each word carries a 32-bit
-// payload and the 32-bit step sequence number that wrote it, so a reader can tell a fresh word
-// from a stale one without any cache-maintenance fence.
+// Exchange buffer geometry of the panel kernel (in 64-bit words).  Every word carries the step
+// tag of the step that wrote it, so a reader can tell a fresh word from a stale one without any
+// cache-maintenance fence.  A slot starts with the two header words (48-bit payload, 16-bit tag
+// in the top bits: the candidate's |value| and row), then the second-largest |value| as two data
+// words, padding, and the candidate row as data words (32-bit payload, tag in the high half).
+// Tags run from 1 to kMaxStepTag; the host clears all exchange words before they would wrap.
 constexpr int kSlotHeaderWords = 8;
+constexpr unsigned kMaxStepTag = 0xFFFFu;
 inline constexpr int slot_words(int nb) { return kSlotHeaderWords + 2 * nb; }
 constexpr int kMaxPanelGroups = 256;
 

@@ -12,6 +12,9 @@
 // columns.  It is held in registers for the whole factorization of the panel: workgroup g owns
 // the R = 64*RPL rows [pb + g*R, pb + (g+1)*R); wave w of a workgroup owns the 16 columns
 // [pb + 16w, pb + 16w + 16); lane l holds rows pb + g*R + 64*r + l (r < RPL) of those columns.
+// Besides its own columns, every wave keeps private copies of the next two pivot columns (the
+// "window"), updated with the same fma as the owner's copy, so the pivot search of the next step
+// needs no hand-over between waves and a step has a single workgroup barrier.
 //
 // One pivot step costs one all-to-all exchange between the workgroups.  Each workgroup publishes
 // its local candidate (|value|, row, second-largest |value|) together with the candidate row's
@@ -71,13 +74,42 @@ __device__ bool ll_get(const uint64_t *p, unsigned seq, unsigned *out, const Pan
   }
 }
 
-__device__ __forceinline__ double join_double(unsigned lo, unsigned hi) {
-  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+// Header words: 48-bit payload, 16-bit step tag on top.  The candidate's |value| (64 bits) and
+// row (32 bits) fit in two words: word 0 = low 48 bits of the value, word 1 = the value's top 16
+// bits above the row.
+__device__ __forceinline__ void hdr_put(uint64_t *p, unsigned seq, double v, int row) {
+  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+  const uint64_t tag = static_cast<uint64_t>(seq) << 48;
+  __hip_atomic_store(p, tag | (b & 0xFFFFFFFFFFFFull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, tag | ((b >> 48) << 32) | static_cast<unsigned>(row), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Candidate order: larger magnitude first, then lower row.  NaN never compares larger.
-__device__ __forceinline__ bool beats(double v, int row, double bv, int brow) {
-  return v > bv || (v == bv && row < brow);
+// Bounded wait for a fresh header; false when the wait expired or another workgroup aborted.
+__device__ bool hdr_get(const uint64_t *p, unsigned seq, double &v, int &row,
+                        const PanelParams &P, long long t0) {
+  for (int it = 1;; ++it) {
+    const uint64_t w0 = ll_load(p), w1 = ll_load(p + 1);
+    if (static_cast<unsigned>(w0 >> 48) == seq && static_cast<unsigned>(w1 >> 48) == seq) {
+      v = __longlong_as_double(
+          static_cast<long long>((w0 & 0xFFFFFFFFFFFFull) | (((w1 >> 32) & 0xFFFFull) << 48)));
+      row = static_cast<int>(static_cast<unsigned>(w1));
+      return true;
+    }
+    if ((it & 31) == 0) {
+      if (__hip_atomic_load(&P.ctl->abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        return false;
+      if (static_cast<long long>(wall_clock64()) - t0 > P.spin_limit) {
+        __hip_atomic_store(&P.ctl->abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        atomicMax(&P.ctl->error, 2);
+        return false;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ double join_double(unsigned lo, unsigned hi) {
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
 __device__ __forceinline__ unsigned lo32(double x) {
@@ -89,6 +121,12 @@ __device__ __forceinline__ unsigned hi32(double x) {
 __device__ __forceinline__ double mk(unsigned lo, unsigned hi) {
   return __longlong_as_double(
       static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Value of x held by lane `src` (wave-uniform result).
+__device__ __forceinline__ double lane_value(double x, int src) {
+  return mk(static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(lo32(x)), src)),
+            static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(hi32(x)), src)));
 }
 
 template <int CTRL>
@@ -161,15 +199,18 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
   const int col0 = wave * 16;
   const int m = P.m;
 
-  __shared__ double s_col[2][R];   // the column being pivoted, for every wave's argmax
-  __shared__ double s_u[NW][16];   // per-wave slice of the pivot row
-  __shared__ double s_k[NW][16];   // per-wave slice of the old row k (for the owner of row p)
-  __shared__ double s_pub[NW][2][16];  // per-wave slices of the candidate row and of row k
-  __shared__ double s_cv[kMaxPanelGroups];
-  __shared__ double s_cv2[kMaxPanelGroups];
-  __shared__ int s_crow[kMaxPanelGroups];
+  // s_win[0..1]: column c+2, handed over by its owner wave during step c (alternating by step);
+  // s_win[2..3]: columns 0 and 1 at the start.
+  __shared__ double s_win[4][R];
+  __shared__ double s_u[NW][16];        // per-wave slice of the pivot row
+  __shared__ double s_k[NW][16];        // per-wave slice of the old row k (owner of row p)
+  __shared__ double s_pub[NW][2][16];   // per-wave slices of the candidate row and of row k
+  __shared__ double s_cv[2][kMaxPanelGroups];   // candidates, alternating by step
+  __shared__ int s_crow[2][kMaxPanelGroups];
   __shared__ int s_fail;
   __shared__ int s_hist[NB];
+  __shared__ double s_g1[NB];   // per step: largest and second-largest |value| (workgroup 0)
+  __shared__ double s_g2[NB];
   __shared__ int s_cur[NB];
   __shared__ int s_out_row[NB];
   __shared__ int s_out_cur[NB];
@@ -185,7 +226,10 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
                     ? P.a[static_cast<size_t>(row) + static_cast<size_t>(P.pb + col) * P.lda]
                     : 0.0;
     }
-    if (wave == 0) s_col[0][64 * r + lane] = v[r][0];
+    if (wave == 0) {
+      s_win[2][64 * r + lane] = v[r][0];
+      s_win[3][64 * r + lane] = v[r][1];
+    }
   }
   if (threadIdx.x == 0) s_fail = 0;
   const bool deferred = P.ctl->deferred != 0;  // a speculative restart after a deferral
@@ -197,6 +241,12 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       P.swap_list[4 * NB] = 0;
     }
     return;
+  }
+  double wa[RPL], wb[RPL];  // private copies of columns c and c+1
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    wa[r] = s_win[2][64 * r + lane];
+    wb[r] = s_win[3][64 * r + lane];
   }
 
   const long long t0 = static_cast<long long>(wall_clock64());
@@ -219,25 +269,21 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
     const int k = P.pb + c;
     const unsigned seq = P.seq_base + static_cast<unsigned>(c) + 1u;
     const int par = static_cast<int>(seq & 1u);
-    const int buf = c & 1;
+    const int cb = c & 1;
     const bool forced = P.positioned && c == 0;
-    const int cc = c & 15;
-    const int owner_wave = c >> 4;
+    const bool k_here = k >= base && k < base + R;
 
-    // Local candidate: every wave reduces this workgroup's part of column c from LDS.  The
-    // winner is the largest magnitude, the lowest row among equal magnitudes (NaN never wins:
-    // max ignores it and it equals nothing), and v2 the largest magnitude of the other rows.
-    double xc[RPL];  // this workgroup's part of column c (kept for the multipliers)
+    // Local candidate from the private copy of column c: the largest magnitude, the lowest row
+    // among equal magnitudes (NaN never wins: max ignores it and it equals nothing), and v2 the
+    // largest magnitude of the other rows.
     double ax[RPL];
     bool ok[RPL];
-#pragma unroll
-    for (int r = 0; r < RPL; ++r) xc[r] = s_col[buf][64 * r + lane];
     double m1 = -1.0;
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
       const int row = base + 64 * r + lane;
       ok[r] = row >= k && row < m && (!forced || row == k);
-      ax[r] = ok[r] ? fabs(xc[r]) : -1.0;
+      ax[r] = ok[r] ? fabs(wa[r]) : -1.0;
       m1 = fmax(m1, ax[r]);
     }
     const double lv1 = wave_max(m1);
@@ -252,33 +298,38 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
     const double lv2 = wave_max(m2);
     DLU_PHASE(0)
 
-    // Publish: header (thread 0), this wave's 16 columns of the candidate row and of row k.
+    // Stage this wave's 16 columns of the candidate row and of row k in LDS (so 32 lanes each
+    // store one exchange word), and hand column c+2 (current through step c-1) to every wave.
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      const int row = base + 64 * r + lane;
+      if (row == lrow) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) s_pub[wave][0][j] = v[r][j];
+      }
+      if (row == k) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) s_pub[wave][1][j] = v[r][j];
+      }
+    }
+    if (c + 2 < width && ((c + 2) >> 4) == wave) {
+      const int j2 = (c + 2) & 15;
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (j == j2) {
+#pragma unroll
+          for (int r = 0; r < RPL; ++r) s_win[cb][64 * r + lane] = v[r][j];
+        }
+    }
+    wave_sync_lds();
+
+    // Publish: 32 lanes store this wave's slice of the candidate row, 32 that of row k; the
+    // last wave stores the header (the polling waves do not wait on it).
     uint64_t *slot = P.slots + (static_cast<size_t>(par) * P.groups + g) * SW;
     uint64_t *rk = P.rowk + static_cast<size_t>(par) * 2 * NB;
-    if (threadIdx.x == 0) {
-      ll_put_double(slot + 0, seq, lv1);
-      ll_put_double(slot + 2, seq, lv2);
-      ll_put(slot + 4, seq, static_cast<unsigned>(lrow));
-    }
     {
-      // Stage the two rows in LDS so 32 lanes each store one exchange word (one store
-      // instruction per wave instead of 32 from a single lane).
-#pragma unroll
-      for (int r = 0; r < RPL; ++r) {
-        const int row = base + 64 * r + lane;
-        if (row == lrow) {
-#pragma unroll
-          for (int j = 0; j < 16; ++j) s_pub[wave][0][j] = v[r][j];
-        }
-        if (row == k) {
-#pragma unroll
-          for (int j = 0; j < 16; ++j) s_pub[wave][1][j] = v[r][j];
-        }
-      }
-      wave_sync_lds();
       const int j = (lane & 31) >> 1, half = lane & 1;
       const bool cand_here = lrow >= base && lrow < base + R;
-      const bool k_here = k >= base && k < base + R;
       if (col0 + j < width && (lane < 32 ? cand_here : k_here)) {
         const double x = s_pub[wave][lane >> 5][j];
         const unsigned pay = half ? hi32(x) : lo32(x);
@@ -286,31 +337,38 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
         ll_put(dstw, seq, pay);
       }
     }
+    if (threadIdx.x == (NW - 1) * 64) {
+      hdr_put(slot, seq, lv1, lrow);
+      ll_put_double(slot + 2, seq, lv2);
+    }
     DLU_PHASE(1)
 
     // Gather every workgroup's header.
     if (threadIdx.x < P.groups) {
       const uint64_t *hs = P.slots + (static_cast<size_t>(par) * P.groups + threadIdx.x) * SW;
-      unsigned w[5];
-      if (!ll_get<5>(hs, seq, w, P, t0)) s_fail = 1;
-      s_cv[threadIdx.x] = join_double(w[0], w[1]);
-      s_cv2[threadIdx.x] = join_double(w[2], w[3]);
-      s_crow[threadIdx.x] = static_cast<int>(w[4]);
+      double hv = 0.0;
+      int hrow = 0;
+      if (!hdr_get(hs, seq, hv, hrow, P, t0)) s_fail = 1;
+      s_cv[cb][threadIdx.x] = hv;
+      s_crow[cb][threadIdx.x] = hrow;
     }
     __syncthreads();
     DLU_PHASE(2)
     if (s_fail) goto leave;
 
     {
+      double wc[RPL];  // private copy of column c+2
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) wc[r] = (c + 2 < width) ? s_win[cb][64 * r + lane] : 0.0;
+
       // Every wave reduces all candidates itself, with the same rule.
       constexpr int GQ = kMaxPanelGroups / 64;
-      double gv[GQ], gw[GQ];
+      double gv[GQ];
       int gq[GQ];
 #pragma unroll
       for (int q = 0; q < GQ; ++q) {  // unconditional LDS reads, then selects
-        gv[q] = s_cv[lane + 64 * q];
-        gw[q] = s_cv2[lane + 64 * q];
-        gq[q] = s_crow[lane + 64 * q];
+        gv[q] = s_cv[cb][lane + 64 * q];
+        gq[q] = s_crow[cb][lane + 64 * q];
       }
       double gm = -1.0;
 #pragma unroll
@@ -329,22 +387,24 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       for (int q = 0; q < GQ; ++q) gr = (gv[q] == g1) ? min(gr, gq[q]) : gr;
       const int p = wave_min(gr);
       const int gwin = (p - P.pb) / R;  // rows are dealt to workgroups in blocks of R
-      double gs = -1.0;
+      double gs = -1.0;  // largest |value| among the other workgroups' candidates
 #pragma unroll
-      for (int q = 0; q < GQ; ++q) gs = fmax(gs, (lane + 64 * q == gwin) ? gw[q] : gv[q]);
-      const double g2 = wave_max(gs);
+      for (int q = 0; q < GQ; ++q) gs = fmax(gs, (lane + 64 * q == gwin) ? -1.0 : gv[q]);
+      gs = wave_max(gs);
       DLU_PHASE(3)
       const bool p_here = p != k && p >= base && p < base + R;
 
       // One exchange round trip for everything this wave needs: its 16 columns of the pivot row
-      // (lanes 0-15), the pivot itself (lane 16), and, when row p lives here, the old row k's
-      // 16 columns (lanes 32-47) and its column-c value (lane 17).
-      const uint64_t *wrow =
-          P.slots + (static_cast<size_t>(par) * P.groups + gwin) * SW + kSlotHeaderWords;
+      // (lanes 0-15), the pivot row at columns c, c+1, c+2 (lanes 16-18), and, when row p lives
+      // here, the old row k at columns c, c+1, c+2 (lanes 19-21) and its 16 columns (32-47);
+      // lane 22 reads the winner's second-largest |value| (for the reported margin).
+      const uint64_t *wslot = P.slots + (static_cast<size_t>(par) * P.groups + gwin) * SW;
+      const uint64_t *wrow = wslot + kSlotHeaderWords;
       const uint64_t *src = nullptr;
       if (lane < 16 && col0 + lane < width) src = wrow + 2 * (col0 + lane);
-      if (lane == 16) src = wrow + 2 * c;
-      if (p_here && lane == 17) src = rk + 2 * c;
+      if (lane >= 16 && lane < 19 && c + lane - 16 < width) src = wrow + 2 * (c + lane - 16);
+      if (p_here && lane >= 19 && lane < 22 && c + lane - 19 < width) src = rk + 2 * (c + lane - 19);
+      if (lane == 22) src = wslot + 2;
       if (p_here && lane >= 32 && lane < 48 && col0 + lane - 32 < width)
         src = rk + 2 * (col0 + lane - 32);
       double got = 0.0;
@@ -355,78 +415,75 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       }
       if (lane < 16) s_u[wave][lane] = got;
       if (lane >= 32 && lane < 48) s_k[wave][lane - 32] = got;
-      const double piv = __shfl(got, 16);
-      const double kc = __shfl(got, 17);
+      const double piv = lane_value(got, 16);
+      const double u1 = lane_value(got, 17), u2 = lane_value(got, 18);
+      const double kc = lane_value(got, 19);
+      const double k1 = lane_value(got, 20), k2 = lane_value(got, 21);
+      const double g2 = fmax(gs, lane_value(got, 22));
       wave_sync_lds();
+      double u[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) u[j] = s_u[wave][j];
       DLU_PHASE(4)
 
-      // Interchange rows k and p (only the lanes holding them), form the multipliers (column c
-      // times the reciprocal of the pivot) and apply the rank-1 update to the panel columns
-      // right of c.  Rows at or above k keep their values bit for bit.
+      // Row k is final now: it is the pivot row (the L part, the pivot and the U part), which
+      // lanes 0-15 of every wave hold, so the workgroup owning row k writes it out directly and
+      // its register copy (like those of the rows above it) is never read again.
+      if (k_here && lane < 16 && col0 + lane < width)
+        P.a[static_cast<size_t>(k) + static_cast<size_t>(P.pb + col0 + lane) * P.lda] = got;
+
+      // Move the old row k into row p (only the lane holding row p), form the multipliers
+      // (column c times the reciprocal of the pivot) and apply the rank-1 update to the panel
+      // columns right of c and to the window.  Every row is updated: those at or above k hold
+      // values that are never read again, and padding rows beyond m are never written back.
       const double recip = 1.0 / piv;
-      double l[RPL];
-      bool below[RPL];
+      const int jc = c - col0;  // this wave's column index of c (may be out of 0..15)
 #pragma unroll
       for (int r = 0; r < RPL; ++r) {
         const int row = base + 64 * r + lane;
-        if (row == k) {  // divergent: only the lane holding row k
-#pragma unroll
-          for (int j = 0; j < 16; ++j) v[r][j] = s_u[wave][j];
-        } else if (p_here && row == p) {  // only the lane holding row p
+        const bool is_p = p_here && row == p;
+        if (is_p) {  // divergent: only the lane holding row p
 #pragma unroll
           for (int j = 0; j < 16; ++j) v[r][j] = s_k[wave][j];
+          wb[r] = k1;
+          wc[r] = k2;
         }
-        below[r] = row > k && row < m;
-        l[r] = ((p_here && row == p) ? kc : xc[r]) * recip;
-      }
-      const int jc = c - col0;  // this wave's column index of c (may be out of 0..15)
+        const double l = (is_p ? kc : wa[r]) * recip;
+        if (jc < 0) {  // wave-uniform: all of this wave's columns lie right of c
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        if (j == jc) {  // wave-uniform: the owner stores the multipliers
+          for (int j = 0; j < 16; ++j) v[r][j] = fma(-l, u[j], v[r][j]);
+        } else if (jc < 16) {
 #pragma unroll
-          for (int r = 0; r < RPL; ++r)
-            if (below[r]) v[r][j] = l[r];
-        } else if (j > jc && col0 + j < width) {  // wave-uniform column skip
-          const double u = s_u[wave][j];
-#pragma unroll
-          for (int r = 0; r < RPL; ++r)
-            if (below[r]) v[r][j] = fma(-l[r], u, v[r][j]);
-        }
-      }
-      // The owner of column c+1 hands its updated column to every wave.
-      if (jc + 1 >= 0 && jc + 1 < 16 && c + 1 < width) {
-#pragma unroll
-        for (int j = 0; j < 16; ++j)
-          if (j == jc + 1) {
-#pragma unroll
-            for (int r = 0; r < RPL; ++r) s_col[buf ^ 1][64 * r + lane] = v[r][j];
+          for (int j = 0; j < 16; ++j) {
+            if (j == jc) v[r][j] = l;
+            if (j > jc) v[r][j] = fma(-l, u[j], v[r][j]);
           }
+        }
+        wb[r] = fma(-l, u1, wb[r]);
+        wc[r] = fma(-l, u2, wc[r]);
       }
       if (g == 0 && threadIdx.x == 0) {
-        if (!forced) P.ipiv[k] = p;  // a forced step keeps the fallback's recorded row
         s_hist[c] = p;
-        double margin = -1.0;
-        if (!forced) {
-          margin = (g1 - fmax(g2, 0.0)) / g1;
-          min_margin = fmin(min_margin, margin);
-          if (margin == 0.0) ++exact_ties;
-          if (margin < 1e-10) ++near_ties;
-        }
-        if (P.margins) P.margins[k] = margin;
+        s_g1[c] = g1;
+        s_g2[c] = g2;
+      }
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) {
+        wa[r] = wb[r];
+        wb[r] = wc[r];
       }
       DLU_PHASE(5)
-      __syncthreads();
-      DLU_PHASE(6)
-      if (s_fail) goto leave;
     }
   }
+  __syncthreads();  // exchange reads of the last step are checked here
+  if (s_fail) goto leave;
 
-finish:
-  // Write the panel back (L below the diagonal, U above, untouched Schur columns after a stop).
+  // Write the rows not yet written back: L multipliers and the Schur columns of rows [stop, m)
+  // (rows [pb, stop) went out as they were finished).
 #pragma unroll
   for (int r = 0; r < RPL; ++r) {
     const int row = base + 64 * r + lane;
-    if (row < m) {
+    if (row >= stop && row < m) {
 #pragma unroll
       for (int j = 0; j < 16; ++j)
         if (col0 + j < width)
@@ -434,9 +491,32 @@ finish:
     }
   }
   if (g == 0) {
+    const int steps = stop - P.pb;
+    // Pivot history and margins of the eliminated steps (a forced step keeps the row the
+    // fallback recorded and has no margin).
+    for (int s = threadIdx.x; s < steps; s += blockDim.x) {
+      if (!(P.positioned && s == 0)) P.ipiv[P.pb + s] = s_hist[s];
+      if (P.margins)
+        P.margins[P.pb + s] =
+            (P.positioned && s == 0) ? -1.0 : (s_g1[s] - fmax(s_g2[s], 0.0)) / s_g1[s];
+    }
+    if (wave == 0) {  // tie statistics of the column steps
+      double lo = 2.0;
+      int near = 0, exact = 0;
+      for (int s0 = 0; s0 < steps; s0 += 64) {
+        const int s = s0 + lane;
+        const bool valid = s < steps && !(P.positioned && s == 0);
+        const double mg = valid ? (s_g1[s] - fmax(s_g2[s], 0.0)) / s_g1[s] : 2.0;
+        lo = fmin(lo, mg);
+        near += __popcll(__ballot(valid && mg < 1e-10));
+        exact += __popcll(__ballot(valid && mg == 0.0));
+      }
+      min_margin = -wave_max(-lo);
+      near_ties = near;
+      exact_ties = exact;
+    }
     // Net row movement of steps [pb, stop) as (dst, src) pairs: rows [pb, stop) first, in order,
     // then every other row whose content changed.  Built by wave 0 from the pivot history.
-    const int steps = stop - P.pb;
     if (wave == 0) {
       for (int i = lane; i < NB; i += 64) s_cur[i] = P.pb + i;
       int out_cnt = 0;
