@@ -176,6 +176,44 @@ __device__ __forceinline__ int wave_min(int x) {
   return static_cast<int>(u);
 }
 
+__device__ __forceinline__ unsigned wave_max_u32(unsigned u) {
+#define DLU_DPP_UMAX(CTRL) u = max(u, dpp<CTRL>(u));
+  DLU_DPP_UMAX(0xB1)
+  DLU_DPP_UMAX(0x4E)
+  DLU_DPP_UMAX(0x141)
+  DLU_DPP_UMAX(0x140)
+#undef DLU_DPP_UMAX
+  {
+    const auto a = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    u = max(a[0], a[1]);
+  }
+  {
+    const auto a = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    u = max(a[0], a[1]);
+  }
+  return u;
+}
+
+// Magnitude keys.  For non-negative doubles the IEEE bit pattern, read as an unsigned 64-bit
+// integer, orders exactly like the value, so |x| is compared through its bits; NaN and rows that
+// may not pivot get key 0, which is also the key of an exact zero (neither can be accepted).
+__device__ __forceinline__ uint64_t mag_key(double x, bool eligible) {
+  const double a = fabs(x);
+  return (eligible && a == a) ? static_cast<uint64_t>(__double_as_longlong(a)) : 0ull;
+}
+__device__ __forceinline__ double key_value(uint64_t key) {
+  return __longlong_as_double(static_cast<long long>(key));
+}
+
+// Wave-wide maximum of a 64-bit key: the high halves first, then the low halves of the lanes
+// holding the largest high half.
+__device__ __forceinline__ uint64_t wave_max_key(uint64_t key) {
+  const unsigned hi = wave_max_u32(static_cast<unsigned>(key >> 32));
+  const unsigned lo =
+      wave_max_u32(static_cast<unsigned>(key >> 32) == hi ? static_cast<unsigned>(key) : 0u);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 // Sixteen columns of one row, kept as a register vector so that a wave-uniform runtime column
 // index lowers to indirect register moves rather than scratch memory.
 typedef double row16_t __attribute__((ext_vector_type(16)));
@@ -205,7 +243,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
   __shared__ double s_u[NW][16];        // per-wave slice of the pivot row
   __shared__ double s_k[NW][16];        // per-wave slice of the old row k (owner of row p)
   __shared__ double s_pub[NW][2][16];   // per-wave slices of the candidate row and of row k
-  __shared__ double s_cv[2][kMaxPanelGroups];   // candidates, alternating by step
+  __shared__ uint64_t s_ck[2][kMaxPanelGroups];  // candidate keys, alternating by step
   __shared__ int s_crow[2][kMaxPanelGroups];
   __shared__ int s_fail;
   __shared__ int s_hist[NB];
@@ -273,29 +311,24 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
     const bool forced = P.positioned && c == 0;
     const bool k_here = k >= base && k < base + R;
 
-    // Local candidate from the private copy of column c: the largest magnitude, the lowest row
-    // among equal magnitudes (NaN never wins: max ignores it and it equals nothing), and v2 the
-    // largest magnitude of the other rows.
-    double ax[RPL];
+    // Local candidate from the private copy of column c: the largest magnitude and the lowest
+    // row among equal magnitudes (NaN never wins).
+    uint64_t key[RPL];
     bool ok[RPL];
-    double m1 = -1.0;
+    uint64_t m1 = 0;
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
       const int row = base + 64 * r + lane;
       ok[r] = row >= k && row < m && (!forced || row == k);
-      ax[r] = ok[r] ? fabs(wa[r]) : -1.0;
-      m1 = fmax(m1, ax[r]);
+      key[r] = mag_key(wa[r], ok[r]);
+      m1 = max(m1, key[r]);
     }
-    const double lv1 = wave_max(m1);
+    const uint64_t lk = wave_max_key(m1);
     int myrow = INT_MAX;
 #pragma unroll
     for (int r = RPL - 1; r >= 0; --r)
-      myrow = (ok[r] && ax[r] == lv1) ? base + 64 * r + lane : myrow;
+      myrow = (ok[r] && key[r] == lk) ? base + 64 * r + lane : myrow;
     const int lrow = wave_min(myrow);
-    double m2 = -1.0;
-#pragma unroll
-    for (int r = 0; r < RPL; ++r) m2 = fmax(m2, (base + 64 * r + lane != lrow) ? ax[r] : -1.0);
-    const double lv2 = wave_max(m2);
     DLU_PHASE(0)
 
     // Stage this wave's 16 columns of the candidate row and of row k in LDS (so 32 lanes each
@@ -337,9 +370,15 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
         ll_put(dstw, seq, pay);
       }
     }
-    if (threadIdx.x == (NW - 1) * 64) {
-      hdr_put(slot, seq, lv1, lrow);
-      ll_put_double(slot + 2, seq, lv2);
+    if (wave == NW - 1) {
+      if (lane == 0) hdr_put(slot, seq, key_value(lk), lrow);
+      // The second-largest |value| of this workgroup's rows, for the reported margins only:
+      // published after the header, off the path every workgroup waits on.
+      uint64_t m2 = 0;
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) m2 = max(m2, (base + 64 * r + lane != lrow) ? key[r] : 0ull);
+      m2 = wave_max_key(m2);
+      if (lane == 0) ll_put_double(slot + 2, seq, key_value(m2));
     }
     DLU_PHASE(1)
 
@@ -349,7 +388,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       double hv = 0.0;
       int hrow = 0;
       if (!hdr_get(hs, seq, hv, hrow, P, t0)) s_fail = 1;
-      s_cv[cb][threadIdx.x] = hv;
+      s_ck[cb][threadIdx.x] = static_cast<uint64_t>(__double_as_longlong(hv));
       s_crow[cb][threadIdx.x] = hrow;
     }
     __syncthreads();
@@ -363,34 +402,37 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
 
       // Every wave reduces all candidates itself, with the same rule.
       constexpr int GQ = kMaxPanelGroups / 64;
-      double gv[GQ];
+      uint64_t gk[GQ];
       int gq[GQ];
 #pragma unroll
       for (int q = 0; q < GQ; ++q) {  // unconditional LDS reads, then selects
-        gv[q] = s_cv[cb][lane + 64 * q];
+        gk[q] = s_ck[cb][lane + 64 * q];
         gq[q] = s_crow[cb][lane + 64 * q];
       }
-      double gm = -1.0;
+      uint64_t gm = 0;
 #pragma unroll
       for (int q = 0; q < GQ; ++q) {
-        gv[q] = lane + 64 * q < P.groups ? gv[q] : -1.0;
-        gm = fmax(gm, gv[q]);
+        gk[q] = lane + 64 * q < P.groups ? gk[q] : 0ull;
+        gm = max(gm, gk[q]);
       }
-      const double g1 = wave_max(gm);
-      if (!(g1 > 0.0)) {  // zero (or NaN-only) column: stop the panel at k
+      const uint64_t g1k = wave_max_key(gm);
+      if (g1k == 0) {  // zero (or NaN-only) column: stop the panel at k
         stop = k;
         zero_col = 1;
         break;
       }
       int gr = INT_MAX;
 #pragma unroll
-      for (int q = 0; q < GQ; ++q) gr = (gv[q] == g1) ? min(gr, gq[q]) : gr;
+      for (int q = 0; q < GQ; ++q) gr = (gk[q] == g1k) ? min(gr, gq[q]) : gr;
       const int p = wave_min(gr);
       const int gwin = (p - P.pb) / R;  // rows are dealt to workgroups in blocks of R
-      double gs = -1.0;  // largest |value| among the other workgroups' candidates
+      double gs = 0.0;  // largest |value| among the other workgroups' candidates (margins only)
+      if (g == 0 && wave == 0) {
+        uint64_t o = 0;
 #pragma unroll
-      for (int q = 0; q < GQ; ++q) gs = fmax(gs, (lane + 64 * q == gwin) ? -1.0 : gv[q]);
-      gs = wave_max(gs);
+        for (int q = 0; q < GQ; ++q) o = max(o, (lane + 64 * q == gwin) ? 0ull : gk[q]);
+        gs = key_value(wave_max_key(o));
+      }
       DLU_PHASE(3)
       const bool p_here = p != k && p >= base && p < base + R;
 
@@ -426,11 +468,12 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       for (int j = 0; j < 16; ++j) u[j] = s_u[wave][j];
       DLU_PHASE(4)
 
-      // Row k is final now: it is the pivot row (the L part, the pivot and the U part), which
-      // lanes 0-15 of every wave hold, so the workgroup owning row k writes it out directly and
-      // its register copy (like those of the rows above it) is never read again.
-      if (k_here && lane < 16 && col0 + lane < width)
-        P.a[static_cast<size_t>(k) + static_cast<size_t>(P.pb + col0 + lane) * P.lda] = got;
+      // Row k is final now: it is the pivot row (the L part, the pivot and the U part).  The
+      // workgroup that published it still holds it staged in LDS and writes it out; the register
+      // copy of row k (like those of the rows above it) is never read again.
+      if (g == gwin && lane < 16 && col0 + lane < width)
+        P.a[static_cast<size_t>(k) + static_cast<size_t>(P.pb + col0 + lane) * P.lda] =
+            s_pub[wave][0][lane];
 
       // Move the old row k into row p (only the lane holding row p), form the multipliers
       // (column c times the reciprocal of the pivot) and apply the rank-1 update to the panel
@@ -464,7 +507,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       }
       if (g == 0 && threadIdx.x == 0) {
         s_hist[c] = p;
-        s_g1[c] = g1;
+        s_g1[c] = key_value(g1k);
         s_g2[c] = g2;
       }
 #pragma unroll
