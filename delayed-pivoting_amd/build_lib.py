@@ -16,7 +16,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(BUILD, "libdlu.so")
-SOURCES = ["dlu_panel.hip", "dlu_blas3.hip", "dlu_pivot.hip", "dlu_getrf.cpp"]
+SOURCES = ["dlu_panel.hip", "dlu_blas3.hip", "dlu_pivot.hip", "dlu_solve.hip", "dlu_getrf.cpp",
+           "dlu_solve.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DLU_OFFLOAD_ARCH", "gfx950")
@@ -29,7 +30,7 @@ FLAGS = [
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    obj = os.path.join(BUILD, src + ".o")  # keeps dlu_solve.hip and dlu_solve.cpp apart
     cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -1,5 +1,5 @@
-// Level-3 kernels of the delayed-pivoting LU on gfx950: the FP64 MFMA trailing update and the
-// triangular solve that forms the U block row.
+// Level-3 kernels of the delayed-pivoting LU on gfx950: the MFMA trailing update and the
+// triangular solve that forms the U block row, in FP64 and FP32.
 //
 // Reference behaviour restated here (delayed_lu/delayed_dgetrf.cpp):
 //   finish_panel: U12 = L11^{-1} A12 (unit lower), A22 -= L21 U12        :61-85
@@ -10,39 +10,34 @@
 namespace dlu {
 namespace {
 
-typedef double double4_t __attribute__((ext_vector_type(4)));
-
 // ---------------------------------------------------------------------------------------------
-// C(M x N) -= A(M x K) * B(K x N), column-major, one leading dimension for all three operands.
+// C(M x N) -= A(M x K) * B(K x N), column-major, leading dimensions ldc, lda, ldb.
 //
 // Workgroup tile BM x BN, 256 threads = 4 waves in a 2 x 2 grid, each wave owns a
-// (BM/2) x (BN/2) block made of 16 x 16 v_mfma_f64_16x16x4_f64 tiles.  K advances in chunks of
-// BK staged through double-buffered LDS; the next chunk's global loads are in flight while the
-// current chunk feeds the MFMA pipe.  Global loads are lane-contiguous (consecutive lanes read
+// (BM/2) x (BN/2) block made of 16 x 16 MFMA tiles (Mfma<T>).  K advances in chunks of BK staged
+// through double-buffered LDS; the next chunk's global loads are in flight while the current
+// chunk feeds the MFMA pipe.  Global loads are lane-contiguous (consecutive lanes read
 // consecutive rows of A, consecutive k of B), 16 bytes per lane when the operands allow it.
-//
-// v_mfma_f64_16x16x4_f64 operand layout (lane l of 64):
-//   A (16 x 4): one double, row l % 16, k l / 16
-//   B (4 x 16): one double, k l / 16, column l % 16
-//   D (16 x 16): four doubles r = 0..3, row l / 16 + 4 * r, column l % 16
 // ---------------------------------------------------------------------------------------------
-template <int BM, int BN, int BK, bool VEC>
-__global__ __launch_bounds__(256, 2) void gemm_sub_kernel(double *__restrict__ C,
-                                                         const double *__restrict__ A,
-                                                         const double *__restrict__ B, int lda,
+template <typename T, int BM, int BN, int BK, bool VEC>
+__global__ __launch_bounds__(256, 2) void gemm_sub_kernel(T *__restrict__ C, int ldc,
+                                                         const T *__restrict__ A, int lda,
+                                                         const T *__restrict__ B, int ldb,
                                                          int M, int N, int K, int tiles_m,
                                                          int tiles_n) {
+  typedef Mfma<T> MF;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int AS = BM + 16;  // A tile k-row stride: consecutive k-rows a fragment read touches
                                // land on opposite LDS bank halves
-  constexpr int BS = BK + 2;   // B tile column stride (16-byte aligned, conflict-free reads)
-  constexpr int VW = VEC ? 2 : 1;             // doubles per global load
+  constexpr int VW = VEC ? 16 / static_cast<int>(sizeof(T)) : 1;  // elements per global load
+  constexpr int BS = BK + 16 / static_cast<int>(sizeof(T));       // B tile column stride
   constexpr int A_LD = BM * BK / (256 * VW);  // loads per thread per chunk
   constexpr int B_LD = BN * BK / (256 * VW);
   constexpr int KS = BK / 4;                  // MFMA k-steps per chunk
-  __shared__ double As[2][BK][AS];
-  __shared__ double Bs[2][BN][BS];
+  static_assert(A_LD >= 1 && B_LD >= 1, "tile too small for the load width");
+  __shared__ T As[2][BK][AS];
+  __shared__ T Bs[2][BN][BS];
 
   // Tile schedule.  Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
   // range of tiles, then walk that range in groups of 8 tile rows so neighbours share L2 data.
@@ -64,9 +59,9 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(double *__restrict__ C
   const int wm = wave & 1, wn = wave >> 1;
 
   // Global loads are branch-free: an out-of-range element is read from a clamped in-range
-  // address and replaced by zero.  On the vector path M and K are even, so a pair is either
-  // wholly inside or wholly outside.
-  typedef double dv_t __attribute__((ext_vector_type(VW)));
+  // address and replaced by zero.  On the vector path M and K are multiples of the vector width,
+  // so a vector is either wholly inside or wholly outside.
+  typedef T dv_t __attribute__((ext_vector_type(VW)));
   dv_t ra[A_LD], rb[B_LD];
   auto load = [&](int k0) {
 #pragma unroll
@@ -76,7 +71,7 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(double *__restrict__ C
       const bool ok = row < M && kk < K;
       const dv_t val = *reinterpret_cast<const dv_t *>(
           A + static_cast<size_t>(ok ? row : 0) + static_cast<size_t>(ok ? kk : 0) * lda);
-      ra[i] = ok ? val : dv_t(0.0);
+      ra[i] = ok ? val : dv_t(T(0));
     }
 #pragma unroll
     for (int i = 0; i < B_LD; ++i) {
@@ -84,8 +79,8 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(double *__restrict__ C
       const int kk = k0 + (e % (BK / VW)) * VW, col = col0 + e / (BK / VW);
       const bool ok = col < N && kk < K;
       const dv_t val = *reinterpret_cast<const dv_t *>(
-          B + static_cast<size_t>(ok ? kk : 0) + static_cast<size_t>(ok ? col : 0) * lda);
-      rb[i] = ok ? val : dv_t(0.0);
+          B + static_cast<size_t>(ok ? kk : 0) + static_cast<size_t>(ok ? col : 0) * ldb);
+      rb[i] = ok ? val : dv_t(T(0));
     }
   };
   auto stash = [&](int buf) {
@@ -101,13 +96,13 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(double *__restrict__ C
     }
   };
 
-  double4_t acc[FM][FN];
+  typename MF::acc_t acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < FN; ++j) acc[i][j] = typename MF::acc_t{T(0), T(0), T(0), T(0)};
 
-  double af[2][FM], bf[2][FN];
+  T af[2][FM], bf[2][FN];
   auto frags = [&](int buf, int kk, int slot) {
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -133,9 +128,7 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(double *__restrict__ C
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s & 1][i], bf[s & 1][j], acc[i][j],
-                                                          0, 0, 0);
+        for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[s & 1][i], bf[s & 1][j], acc[i][j]);
       __builtin_amdgcn_sched_barrier(0);
     }
     if (kt + 1 < nk) stash(cur ^ 1);
@@ -150,9 +143,9 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(double *__restrict__ C
       if (col >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = row0 + wm * WM + 16 * i + (lane >> 4) + 4 * r;
+        const int row = row0 + wm * WM + 16 * i + MF::acc_row(lane, r);
         if (row < M) {
-          double *cp = C + static_cast<size_t>(row) + static_cast<size_t>(col) * lda;
+          T *cp = C + static_cast<size_t>(row) + static_cast<size_t>(col) * ldc;
           *cp = *cp - acc[i][j][r];
         }
       }
@@ -167,20 +160,21 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(double *__restrict__ C
 // The strip's touched rows are gathered (all reads before any write), displaced rows are written
 // back in place, and rows [pb, pb+w) are solved in LDS:
 //   X := L11^{-1} X  with L11 = A(pb:pb+w, pb:pb+w) unit lower,
-// by 16-row blocks on FP64 MFMA: X_b := inv(L_bb) X_b (the 16 x 16 diagonal inverses are formed
-// by every workgroup for itself), then X_rest -= L(rest, b) X_b with L's column slice staged in
+// by 16-row blocks on MFMA: X_b := inv(L_bb) X_b (the 16 x 16 diagonal inverses are formed by
+// every workgroup for itself), then X_rest -= L(rest, b) X_b with L's column slice staged in
 // LDS.  Rows at and beyond w are zero-padded and never written back.
 // ---------------------------------------------------------------------------------------------
-template <int NB>
-__global__ __launch_bounds__(256) void swap_trsm_kernel(double *__restrict__ a, int lda, int pb,
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void swap_trsm_kernel(T *__restrict__ a, int lda, int pb,
                                                        int w, int c0, int c1,
                                                        const int *__restrict__ swap_list) {
+  typedef Mfma<T> MF;
   constexpr int NS = 16;
-  constexpr int XS = NS + 2;  // row stride of X (doubles)
-  __shared__ double X[NB][XS];
-  __shared__ double E[NB][NS + 1];
-  __shared__ double invL[NB / 16][16][17];
-  __shared__ double Ls[NB][17];  // rows [r0+16, w) of L's current 16-column slice
+  constexpr int XS = NS + 2;  // row stride of X
+  __shared__ T X[NB][XS];
+  __shared__ T E[NB][NS + 1];
+  __shared__ T invL[NB / 16][16][17];
+  __shared__ T Ls[NB][17];  // rows [r0+16, w) of L's current 16-column slice
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int cnt = swap_list[4 * NB];
@@ -190,37 +184,37 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(double *__restrict__ a, 
   const int cs = c0 + blockIdx.x * NS;
   const int ncol = (c1 - cs) < NS ? (c1 - cs) : NS;
   const int nblk = (w + 15) / 16;
-  const double *L = a + static_cast<size_t>(pb) + static_cast<size_t>(pb) * lda;
+  const T *L = a + static_cast<size_t>(pb) + static_cast<size_t>(pb) * lda;
 
   // Gather every touched row of the strip before anything is written (rows are contiguous in
   // lane order for a fixed column).
   for (int idx = t; idx < cnt * NS; idx += 256) {
     const int i = idx % cnt, col = idx / cnt;
-    const double val =
-        col < ncol ? a[static_cast<size_t>(src[i]) + static_cast<size_t>(cs + col) * lda] : 0.0;
+    const T val =
+        col < ncol ? a[static_cast<size_t>(src[i]) + static_cast<size_t>(cs + col) * lda] : T(0);
     if (i < w)
       X[i][col] = val;
     else
       E[i - w][col] = val;
   }
-  for (int idx = t; idx < (nblk * 16 - w) * NS; idx += 256) X[w + idx / NS][idx % NS] = 0.0;
+  for (int idx = t; idx < (nblk * 16 - w) * NS; idx += 256) X[w + idx / NS][idx % NS] = T(0);
 
   // Inverses of the diagonal blocks: thread (blk, col) solves L_bb x = e_col.
   if (t < 16 * nblk) {
     const int blk = t >> 4, col = t & 15;
     const int r0 = blk * 16;
     const int nr = (w - r0) < 16 ? (w - r0) : 16;
-    double x[16];
+    T x[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      double sum = (r == col) ? 1.0 : 0.0;
+      T sum = (r == col) ? T(1) : T(0);
       if (r > col && r < nr) {
 #pragma unroll
         for (int q = 0; q < 16; ++q)
           if (q >= col && q < r)
             sum -= L[static_cast<size_t>(r0 + r) + static_cast<size_t>(r0 + q) * lda] * x[q];
       }
-      x[r] = (r < nr) ? sum : 0.0;
+      x[r] = (r < nr) ? sum : T(0);
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) invL[blk][r][col] = x[r];
@@ -241,34 +235,34 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(double *__restrict__ a, 
       const int row = r0 + 16 + rr, colq = r0 + q;
       Ls[rr][q] = (row < w && colq < w)
                       ? L[static_cast<size_t>(row) + static_cast<size_t>(colq) * lda]
-                      : 0.0;
+                      : T(0);
     }
     if (wave == 0) {
-      double4_t d = {0.0, 0.0, 0.0, 0.0};
+      typename MF::acc_t d = {T(0), T(0), T(0), T(0)};
 #pragma unroll
       for (int k4 = 0; k4 < 16; k4 += 4) {
-        const double av = invL[blk][lr][k4 + lk];
-        const double bv = X[r0 + k4 + lk][lr];
-        d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
+        const T av = invL[blk][lr][k4 + lk];
+        const T bv = X[r0 + k4 + lk][lr];
+        d = MF::mma(av, bv, d);
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) X[r0 + lk + 4 * r][lr] = d[r];
+      for (int r = 0; r < 4; ++r) X[r0 + MF::acc_row(lane, r)][lr] = d[r];
     }
     __syncthreads();
     // X(rows below) -= L(rows below, block) * X_b, one 16 x 16 tile per wave at a time.
     for (int tb = wave; tb < below / 16; tb += 4) {
       const int rt = r0 + 16 + 16 * tb;
-      double4_t d;
+      typename MF::acc_t d;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) d[r] = X[rt + lk + 4 * r][lr];
+      for (int r = 0; r < 4; ++r) d[r] = X[rt + MF::acc_row(lane, r)][lr];
 #pragma unroll
       for (int k4 = 0; k4 < 16; k4 += 4) {
-        const double av = -Ls[16 * tb + lr][k4 + lk];
-        const double bv = X[r0 + k4 + lk][lr];
-        d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
+        const T av = -Ls[16 * tb + lr][k4 + lk];
+        const T bv = X[r0 + k4 + lk][lr];
+        d = MF::mma(av, bv, d);
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) X[rt + lk + 4 * r][lr] = d[r];
+      for (int r = 0; r < 4; ++r) X[rt + MF::acc_row(lane, r)][lr] = d[r];
     }
     __syncthreads();
   }
@@ -279,43 +273,56 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(double *__restrict__ a, 
   }
 }
 
-template <int BM, int BN, int BK>
-hipError_t launch_gemm_tile(double *c, const double *a, const double *b, int lda, int M, int N,
+template <typename T, int BM, int BN, int BK>
+hipError_t launch_gemm_tile(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
                             int K, hipStream_t s) {
   const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
-  // 16-byte loads: 16-byte aligned origins, even leading dimension, even M and K.
-  const bool vec = (lda % 2 == 0) && (M % 2 == 0) && (K % 2 == 0) &&
+  // 16-byte loads: 16-byte aligned origins, leading dimensions, M and K multiples of the width.
+  constexpr int vw = 16 / static_cast<int>(sizeof(T));
+  const bool vec = (lda % vw == 0) && (ldb % vw == 0) && (M % vw == 0) && (K % vw == 0) &&
                    (reinterpret_cast<uintptr_t>(a) % 16 == 0) &&
                    (reinterpret_cast<uintptr_t>(b) % 16 == 0);
   if (vec)
-    hipLaunchKernelGGL((gemm_sub_kernel<BM, BN, BK, true>), dim3(tm * tn), dim3(256), 0, s, c, a,
-                       b, lda, M, N, K, tm, tn);
+    hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK, true>), dim3(tm * tn), dim3(256), 0, s, c,
+                       ldc, a, lda, b, ldb, M, N, K, tm, tn);
   else
-    hipLaunchKernelGGL((gemm_sub_kernel<BM, BN, BK, false>), dim3(tm * tn), dim3(256), 0, s, c, a,
-                       b, lda, M, N, K, tm, tn);
+    hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK, false>), dim3(tm * tn), dim3(256), 0, s,
+                       c, ldc, a, lda, b, ldb, M, N, K, tm, tn);
   return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_gemm_sub(double *c, const double *a, const double *b, int lda, int M, int N,
+template <typename T>
+hipError_t launch_gemm_sub(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
                            int K, hipStream_t s) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
-  if (gemm_uses_large_tiles(M, N)) return launch_gemm_tile<128, 128, 16>(c, a, b, lda, M, N, K, s);
-  return launch_gemm_tile<64, 64, 16>(c, a, b, lda, M, N, K, s);
+  if (gemm_uses_large_tiles(M, N))
+    return launch_gemm_tile<T, 128, 128, 16>(c, ldc, a, lda, b, ldb, M, N, K, s);
+  return launch_gemm_tile<T, 64, 64, 16>(c, ldc, a, lda, b, ldb, M, N, K, s);
 }
 
-hipError_t launch_swap_trsm(double *a, int lda, int pb, int w, int c0, int c1,
-                            const int *swap_list, int nb, hipStream_t s) {
+template <typename T>
+hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const int *swap_list,
+                            int nb, hipStream_t s) {
   if (c1 <= c0 || w <= 0) return hipSuccess;
   const dim3 grid((c1 - c0 + 15) / 16), block(256);
   switch (nb) {
-    case 64: hipLaunchKernelGGL(swap_trsm_kernel<64>, grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
-    case 128: hipLaunchKernelGGL(swap_trsm_kernel<128>, grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
-    case 256: hipLaunchKernelGGL(swap_trsm_kernel<256>, grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
+    case 64: hipLaunchKernelGGL((swap_trsm_kernel<T, 64>), grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
+    case 128: hipLaunchKernelGGL((swap_trsm_kernel<T, 128>), grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
+    case 256: hipLaunchKernelGGL((swap_trsm_kernel<T, 256>), grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
+
+#define DLU_BLAS3_INSTANTIATE(T)                                                                 \
+  template hipError_t launch_gemm_sub<T>(T *, int, const T *, int, const T *, int, int, int, int, \
+                                         hipStream_t);                                           \
+  template hipError_t launch_swap_trsm<T>(T *, int, int, int, int, int, const int *, int,        \
+                                          hipStream_t);
+DLU_BLAS3_INSTANTIATE(float)
+DLU_BLAS3_INSTANTIATE(double)
+#undef DLU_BLAS3_INSTANTIATE
 
 }  // namespace dlu

@@ -11,7 +11,7 @@
 //     the panel stopped, and uses it only to size the next launches.
 // Look-ahead: after panel j, the columns of panel j+1 are updated and factored on the caller's
 // stream while the update of the far columns runs on a second stream of the same device.
-#include "../../include/dlu.h"
+#include "dlu.h"
 #include "dlu_internal.hpp"
 
 #include <hip/hip_runtime.h>
@@ -27,17 +27,17 @@ namespace dlu {
 namespace {
 
 // delayed_lu_cuda/delayed_dgetrf_cuda.cu:370-383: lda*n*8 must not overflow size_t.
-bool matrix_bytes(int lda, int n, size_t &bytes) {
+bool matrix_bytes(int lda, int n, size_t &bytes, size_t elem = sizeof(double)) {
   const size_t cols = static_cast<size_t>(n), lead = static_cast<size_t>(lda);
   if (cols != 0 && lead > std::numeric_limits<size_t>::max() / cols) return false;
   const size_t elems = lead * cols;
-  if (elems > std::numeric_limits<size_t>::max() / sizeof(double)) return false;
-  bytes = elems * sizeof(double);
+  if (elems > std::numeric_limits<size_t>::max() / elem) return false;
+  bytes = elems * elem;
   return true;
 }
 
 // delayed_lu_cuda/delayed_dgetrf_cuda.cu:385-397 and delayed_lu/delayed_dgetrf.cpp:132-142.
-bool valid_arguments(int m, int n, const double *a, int lda, const int *ipiv, const int *jpiv) {
+bool valid_arguments(int m, int n, const void *a, int lda, const int *ipiv, const int *jpiv) {
   if (m < 0 || n < 0 || lda < std::max(1, m)) return false;
   const int steps = std::min(m, n);
   if ((m > 0 && n > 0 && a == nullptr) || (steps > 0 && (ipiv == nullptr || jpiv == nullptr)))
@@ -165,14 +165,18 @@ hipError_t stream_pair(Workspace &w, int panel_cus, Workspace::Streams *&out) {
   return hipSuccess;
 }
 
-inline double *at(double *a, int lda, int i, int j) {
+template <typename T>
+inline T *at(T *a, int lda, int i, int j) {
   return a + static_cast<size_t>(i) + static_cast<size_t>(j) * static_cast<size_t>(lda);
 }
 
+// One factorization of a column-major T matrix (T = double, or float for the FP32 factor of the
+// mixed-precision solver).
+template <typename T>
 struct Run {
   Workspace &w;
   int m, n, lda, steps, nb;
-  double *a;
+  T *a;
   hipStream_t s;  // the caller's stream
   long long spin;
   bool profile = false;
@@ -232,7 +236,7 @@ struct Run {
   hipError_t panel(int pb, int pe, int positioned, int parity) {
     int groups = 0, rpl = 0;
     if (!panel_shape(pb, groups, rpl)) return hipErrorInvalidValue;
-    PanelParams p{};
+    PanelParams<T> p{};
     p.a = a;
     p.lda = lda;
     p.m = m;
@@ -282,8 +286,8 @@ struct Run {
     const double flops = 2.0 * (m - r0) * static_cast<double>(c1 - c0) * wdt;
     std::vector<Timed> &tv = gemm_uses_large_tiles(m - r0, c1 - c0) ? t_update_big : t_update;
     if (profile && (e = mark(tv, st, flops)) != hipSuccess) return e;
-    e = launch_gemm_sub(at(a, lda, r0, c0), at(a, lda, r0, pb), at(a, lda, pb, c0), lda, m - r0,
-                        c1 - c0, wdt, st);
+    e = launch_gemm_sub<T>(at(a, lda, r0, c0), lda, at(a, lda, r0, pb), lda, at(a, lda, pb, c0),
+                           lda, m - r0, c1 - c0, wdt, st);
     if (e == hipSuccess && profile) e = close(tv, st);
     return e;
   }
@@ -364,6 +368,8 @@ struct Run {
   }
 };
 
+}  // namespace
+
 dlu_status to_status(hipError_t e) {
   if (e == hipSuccess) return DLU_SUCCESS;
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return DLU_ALLOCATION_FAILED;
@@ -371,7 +377,8 @@ dlu_status to_status(hipError_t e) {
   return DLU_RUNTIME_FAILURE;
 }
 
-dlu_status getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv, int *info,
+template <typename T>
+dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int *info,
                         hipStream_t stream, const dlu_options *opt, dlu_report *rep,
                         double *margins) {
   if (info == nullptr) return DLU_INVALID_ARGUMENT;
@@ -407,7 +414,7 @@ dlu_status getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv
     *info = -1;
     return to_status(e);
   }
-  Run run{w, m, n, lda, steps, nb, d_a, stream, spin};
+  Run<T> run{w, m, n, lda, steps, nb, d_a, stream, spin};
   run.profile = opt && opt->profile;
   if (opt && opt->rows_per_lane) {
     if (opt->rows_per_lane != 1 && opt->rows_per_lane != 2 && opt->rows_per_lane != 4) {
@@ -484,14 +491,14 @@ dlu_status getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv
     if (run.profile) {
       rep->large_update_launches = static_cast<int>(run.t_update_big.size());
       rep->large_update_seconds =
-          Run::total(run.t_update_big, &rep->large_update_flops);
+          Run<T>::total(run.t_update_big, &rep->large_update_flops);
       rep->update_launches = static_cast<int>(run.t_update.size()) + rep->large_update_launches;
-      rep->update_seconds = Run::total(run.t_update, &rep->update_flops) + rep->large_update_seconds;
+      rep->update_seconds = Run<T>::total(run.t_update, &rep->update_flops) + rep->large_update_seconds;
       rep->update_flops += rep->large_update_flops;
       rep->panel_launches = static_cast<int>(run.t_panel.size());
-      rep->panel_seconds = Run::total(run.t_panel, nullptr);
+      rep->panel_seconds = Run<T>::total(run.t_panel, nullptr);
       rep->solve_launches = static_cast<int>(run.t_solve.size());
-      rep->solve_seconds = Run::total(run.t_solve, nullptr);
+      rep->solve_seconds = Run<T>::total(run.t_solve, nullptr);
       unsigned long long ticks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       if (hipMemcpy(ticks, w.d_phase, sizeof ticks, hipMemcpyDeviceToHost) == hipSuccess)
         for (int i = 0; i < 8; ++i) rep->panel_phase_seconds[i] = ticks[i] * 1e-8;  // 100 MHz
@@ -528,7 +535,12 @@ dlu_status getrf_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv, in
   return DLU_SUCCESS;
 }
 
-}  // namespace
+template dlu_status getrf_device<float>(int, int, float *, int, int *, int *, int *, hipStream_t,
+                                        const dlu_options *, dlu_report *, double *);
+template dlu_status getrf_device<double>(int, int, double *, int, int *, int *, int *,
+                                         hipStream_t, const dlu_options *, dlu_report *,
+                                         double *);
+
 }  // namespace dlu
 
 extern "C" {
@@ -542,6 +554,19 @@ dlu_status dlu_getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *
 dlu_status dlu_getrf_device_ex(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
                                int *info, void *stream, const dlu_options *options,
                                dlu_report *report, double *margins) {
+  return dlu::getrf_device(m, n, d_a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream),
+                           options, report, margins);
+}
+
+dlu_status dlu_sgetrf_device(int m, int n, float *d_a, int lda, int *ipiv, int *jpiv, int *info,
+                             void *stream) {
+  return dlu::getrf_device(m, n, d_a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream),
+                           nullptr, nullptr, nullptr);
+}
+
+dlu_status dlu_sgetrf_device_ex(int m, int n, float *d_a, int lda, int *ipiv, int *jpiv,
+                                int *info, void *stream, const dlu_options *options,
+                                dlu_report *report, double *margins) {
   return dlu::getrf_device(m, n, d_a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream),
                            options, report, margins);
 }
@@ -588,7 +613,7 @@ dlu_status dlu_gemm_sub_device(int M, int N, int K, double *C, const double *A, 
       (M > 0 && K > 0 && A == nullptr) || (K > 0 && N > 0 && B == nullptr) || ld < K)
     return DLU_INVALID_ARGUMENT;
   return dlu::to_status(
-      dlu::launch_gemm_sub(C, A, B, ld, M, N, K, static_cast<hipStream_t>(stream)));
+      dlu::launch_gemm_sub<double>(C, ld, A, ld, B, ld, M, N, K, static_cast<hipStream_t>(stream)));
 }
 
 }  // extern "C"

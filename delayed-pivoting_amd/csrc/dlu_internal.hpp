@@ -10,6 +10,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "dlu.h"
+
 namespace dlu {
 
 // Control block written on the device.  One per workspace.
@@ -41,13 +43,37 @@ constexpr unsigned kMaxStepTag = 0xFFFFu;
 inline constexpr int slot_words(int nb) { return kSlotHeaderWords + 2 * nb; }
 constexpr int kMaxPanelGroups = 256;
 
+// 16 x 16 x 4 MFMA tiles in either precision (gfx950: v_mfma_f64_16x16x4_f64 and
+// v_mfma_f32_16x16x4_f32).  Operand layout, lane l of 64: A (16 x 4) holds row l % 16, k l / 16;
+// B (4 x 16) holds k l / 16, column l % 16.  The 16 x 16 result is four values per lane in
+// column l % 16; their rows differ between the two instructions (acc_row).
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int acc_row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int acc_row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
 // A panel's net row movement: [0, 2nb) destination rows, [2nb, 4nb) source rows, [4nb] count.
 // Two such lists alternate between consecutive panels so the far update of panel j can still
 // read its list while panel j+1 writes the next one.
 inline constexpr int swap_list_ints(int nb) { return 4 * nb + 4; }
 
+template <typename T>
 struct PanelParams {
-  double *a;
+  T *a;
   int lda;
   int m;
   int pb;         // first column (and row) of the panel
@@ -65,35 +91,82 @@ struct PanelParams {
   unsigned long long *phase_ticks;  // optional [8] per-phase tick totals (workgroup 0)
 };
 
-// Launchers (return hipSuccess or the launch error).
-hipError_t launch_panel(const PanelParams &p, int nb, int rows_per_lane, hipStream_t s);
+// Launchers (return hipSuccess or the launch error).  T is float or double; both are
+// instantiated in the translation unit that defines each launcher.
+template <typename T>
+hipError_t launch_panel(const PanelParams<T> &p, int nb, int rows_per_lane, hipStream_t s);
 
 // Right-side interchange + triangular solve: rows [pb, pb+w) of columns [c0, c1) become
 // U12 = L11^{-1} * (rows after interchange); rows displaced below pb+w are written directly.
-hipError_t launch_swap_trsm(double *a, int lda, int pb, int w, int c0, int c1,
-                            const int *swap_list, int nb, hipStream_t s);
+template <typename T>
+hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const int *swap_list,
+                            int nb, hipStream_t s);
 
-// C(M x N) -= A(M x K) * B(K x N), column-major, all with leading dimension lda.
+// C(M x N) -= A(M x K) * B(K x N), column-major with leading dimensions ldc, lda, ldb.
 // Large problems use 128 x 128 tiles, small ones 64 x 64 (gemm_uses_large_tiles tells which).
 inline bool gemm_uses_large_tiles(int M, int N) {
   return static_cast<long long>((M + 127) / 128) * ((N + 127) / 128) >= 512;
 }
-hipError_t launch_gemm_sub(double *c, const double *a, const double *b, int lda, int M, int N,
+template <typename T>
+hipError_t launch_gemm_sub(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
                            int K, hipStream_t s);
 
 // Row interchanges of the last panel applied to columns [c0, c1) (left of the panel).
-hipError_t launch_row_interchange(double *a, int lda, int c0, int c1, const int *swap_list,
-                                  int nb, hipStream_t s);
+template <typename T>
+hipError_t launch_row_interchange(T *a, int lda, int c0, int c1, const int *swap_list, int nb,
+                                  hipStream_t s);
 
 // Identity swap histories and "unreached" margins for a new factorization.
 hipError_t launch_init_steps(int *ipiv, int *jpiv, double *margins, int steps, hipStream_t s);
 
 // Full-pivot search over the active block A(k:m-1, k:n-1); result in ctl->best_*.
-hipError_t launch_active_search(const double *a, int lda, int m, int n, int k, double *partial_v,
+template <typename T>
+hipError_t launch_active_search(const T *a, int lda, int m, int n, int k, double *partial_v,
                                 unsigned long long *partial_o, Control *ctl, hipStream_t s);
 
 // Apply the full-pivot decision at step k (row swap, then column swap) or mark the deferral.
-hipError_t launch_fallback_apply(double *a, int lda, int m, int n, int k, int *ipiv, int *jpiv,
+template <typename T>
+hipError_t launch_fallback_apply(T *a, int lda, int m, int n, int k, int *ipiv, int *jpiv,
                                  Control *ctl, hipStream_t s);
+
+// ---- Solve-side launchers (dlu_solve.hip) ----------------------------------------------------
+
+// dst(i, j) = src(perm[i], j) for i < n, j < nrhs (column-major; perm on the device).
+template <typename T>
+hipError_t launch_gather_rows(const T *src, int lds, T *dst, int ldd, const int *perm, int n,
+                              int nrhs, hipStream_t s);
+
+// Rows [r0, r0+w) of X (w <= 128) := T^{-1} X(rows), T the diagonal block of the factor at
+// (r0, r0): unit lower (upper = false) or non-unit upper (upper = true).
+template <typename T>
+hipError_t launch_block_trsv(const T *lu, int lda, int r0, int w, T *x, int ldx, int nrhs,
+                             bool upper, hipStream_t s);
+
+// R = B - A X in FP64 (A n x n, X and B n x nrhs).
+hipError_t launch_residual(const double *a, int lda, const double *x, int ldx, const double *b,
+                           int ldb, double *r, int ldr, int n, int nrhs, hipStream_t s);
+
+// out[0] = max_i sum_j |A(i, j)| (the infinity norm of an n x n matrix); work holds 32*n
+// doubles; out[0] must be zeroed first.
+hipError_t launch_norm_inf_matrix(const double *a, int lda, int n, double *work, double *out,
+                                  hipStream_t s);
+
+// out[j] = max_i |X(i, j)| for j < nrhs (out must be zeroed first; accumulates with max).
+hipError_t launch_norm_inf_columns(const double *x, int ldx, int n, int nrhs, double *out,
+                                   hipStream_t s);
+
+// Precision conversions (column-major, n x ncol): dst = float(src); x += double(d).
+hipError_t launch_to_float(const double *src, int lds, float *dst, int ldd, int n, int ncol,
+                           hipStream_t s);
+template <typename T>
+hipError_t launch_add_correction(double *x, int ldx, const T *d, int ldd, int n, int ncol,
+                                 hipStream_t s);
+
+// ---- Host drivers shared between translation units -------------------------------------------
+dlu_status to_status(hipError_t e);
+template <typename T>
+dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int *info,
+                        hipStream_t stream, const dlu_options *opt, dlu_report *rep,
+                        double *margins);
 
 }  // namespace dlu

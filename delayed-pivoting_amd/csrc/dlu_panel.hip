@@ -50,8 +50,8 @@ __device__ __forceinline__ uint64_t ll_load(const uint64_t *p) {
 
 // Bounded wait for `count` consecutive exchange words tagged `seq`; payloads land in out[].
 // Returns false when the wait expired or another workgroup aborted.
-template <int COUNT>
-__device__ bool ll_get(const uint64_t *p, unsigned seq, unsigned *out, const PanelParams &P,
+template <int COUNT, typename Params>
+__device__ bool ll_get(const uint64_t *p, unsigned seq, unsigned *out, const Params &P,
                        long long t0) {
   for (int it = 1;; ++it) {
     bool fresh = true;
@@ -86,8 +86,9 @@ __device__ __forceinline__ void hdr_put(uint64_t *p, unsigned seq, double v, int
 }
 
 // Bounded wait for a fresh header; false when the wait expired or another workgroup aborted.
-__device__ bool hdr_get(const uint64_t *p, unsigned seq, double &v, int &row,
-                        const PanelParams &P, long long t0) {
+template <typename Params>
+__device__ bool hdr_get(const uint64_t *p, unsigned seq, double &v, int &row, const Params &P,
+                        long long t0) {
   for (int it = 1;; ++it) {
     const uint64_t w0 = ll_load(p), w1 = ll_load(p + 1);
     if (static_cast<unsigned>(w0 >> 48) == seq && static_cast<unsigned>(w1 >> 48) == seq) {
@@ -127,6 +128,31 @@ __device__ __forceinline__ double mk(unsigned lo, unsigned hi) {
 __device__ __forceinline__ double lane_value(double x, int src) {
   return mk(static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(lo32(x)), src)),
             static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(hi32(x)), src)));
+}
+__device__ __forceinline__ float lane_value(float x, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+}
+
+// An element travels as Words<T>::n exchange words of 32-bit payload each.
+template <typename T>
+struct Words;
+template <>
+struct Words<double> {
+  static constexpr int n = 2;
+  static __device__ __forceinline__ unsigned part(double x, int i) { return i ? hi32(x) : lo32(x); }
+  static __device__ __forceinline__ double join(const unsigned *w) { return mk(w[0], w[1]); }
+};
+template <>
+struct Words<float> {
+  static constexpr int n = 1;
+  static __device__ __forceinline__ unsigned part(float x, int) { return __float_as_uint(x); }
+  static __device__ __forceinline__ float join(const unsigned *w) { return __uint_as_float(w[0]); }
+};
+
+template <typename T>
+__device__ __forceinline__ void ll_put_value(uint64_t *p, unsigned seq, T v) {
+#pragma unroll
+  for (int i = 0; i < Words<T>::n; ++i) ll_put(p + i, seq, Words<T>::part(v, i));
 }
 
 template <int CTRL>
@@ -201,6 +227,10 @@ __device__ __forceinline__ uint64_t mag_key(double x, bool eligible) {
   const double a = fabs(x);
   return (eligible && a == a) ? static_cast<uint64_t>(__double_as_longlong(a)) : 0ull;
 }
+// Float magnitudes widen to double exactly, so both precisions share the 64-bit key.
+__device__ __forceinline__ uint64_t mag_key(float x, bool eligible) {
+  return mag_key(static_cast<double>(x), eligible);
+}
 __device__ __forceinline__ double key_value(uint64_t key) {
   return __longlong_as_double(static_cast<long long>(key));
 }
@@ -214,18 +244,18 @@ __device__ __forceinline__ uint64_t wave_max_key(uint64_t key) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
-// Sixteen columns of one row, kept as a register vector so that a wave-uniform runtime column
-// index lowers to indirect register moves rather than scratch memory.
-typedef double row16_t __attribute__((ext_vector_type(16)));
-
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int NB, int RPL>
-__global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
+template <typename T, int NB, int RPL>
+__global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
+  // Sixteen columns of one row, kept as a register vector so that a wave-uniform runtime column
+  // index lowers to indirect register moves rather than scratch memory.
+  typedef T row16_t __attribute__((ext_vector_type(16)));
+  constexpr int WPV = Words<T>::n;  // exchange words per element
   constexpr int R = 64 * RPL;
   constexpr int SW = slot_words(NB);
   constexpr int NW = NB / 16;
@@ -239,10 +269,10 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
 
   // s_win[0..1]: column c+2, handed over by its owner wave during step c (alternating by step);
   // s_win[2..3]: columns 0 and 1 at the start.
-  __shared__ double s_win[4][R];
-  __shared__ double s_u[NW][16];        // per-wave slice of the pivot row
-  __shared__ double s_k[NW][16];        // per-wave slice of the old row k (owner of row p)
-  __shared__ double s_pub[NW][2][16];   // per-wave slices of the candidate row and of row k
+  __shared__ T s_win[4][R];
+  __shared__ T s_u[NW][16];        // per-wave slice of the pivot row
+  __shared__ T s_k[NW][16];        // per-wave slice of the old row k (owner of row p)
+  __shared__ T s_pub[NW][2][16];   // per-wave slices of the candidate row and of row k
   __shared__ uint64_t s_ck[2][kMaxPanelGroups];  // candidate keys, alternating by step
   __shared__ int s_crow[2][kMaxPanelGroups];
   __shared__ int s_fail;
@@ -262,7 +292,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       const int col = col0 + j;
       v[r][j] = (row < m && col < width)
                     ? P.a[static_cast<size_t>(row) + static_cast<size_t>(P.pb + col) * P.lda]
-                    : 0.0;
+                    : T(0);
     }
     if (wave == 0) {
       s_win[2][64 * r + lane] = v[r][0];
@@ -280,7 +310,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
     }
     return;
   }
-  double wa[RPL], wb[RPL];  // private copies of columns c and c+1
+  T wa[RPL], wb[RPL];  // private copies of columns c and c+1
 #pragma unroll
   for (int r = 0; r < RPL; ++r) {
     wa[r] = s_win[2][64 * r + lane];
@@ -356,18 +386,18 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
     }
     wave_sync_lds();
 
-    // Publish: 32 lanes store this wave's slice of the candidate row, 32 that of row k; the
-    // last wave stores the header (the polling waves do not wait on it).
+    // Publish: lanes 0-31 store this wave's slice of the candidate row, lanes 32-63 that of
+    // row k, one exchange word each; the last wave stores the header (the polling waves do not
+    // wait on it).
     uint64_t *slot = P.slots + (static_cast<size_t>(par) * P.groups + g) * SW;
     uint64_t *rk = P.rowk + static_cast<size_t>(par) * 2 * NB;
     {
-      const int j = (lane & 31) >> 1, half = lane & 1;
+      const int j = (lane & 31) / WPV, part = lane % WPV;
       const bool cand_here = lrow >= base && lrow < base + R;
-      if (col0 + j < width && (lane < 32 ? cand_here : k_here)) {
-        const double x = s_pub[wave][lane >> 5][j];
-        const unsigned pay = half ? hi32(x) : lo32(x);
-        uint64_t *dstw = (lane < 32 ? slot + kSlotHeaderWords : rk) + 2 * (col0 + j) + half;
-        ll_put(dstw, seq, pay);
+      if (j < 16 && col0 + j < width && (lane < 32 ? cand_here : k_here)) {
+        const T x = s_pub[wave][lane >> 5][j];
+        uint64_t *dstw = (lane < 32 ? slot + kSlotHeaderWords : rk) + WPV * (col0 + j) + part;
+        ll_put(dstw, seq, Words<T>::part(x, part));
       }
     }
     if (wave == NW - 1) {
@@ -378,7 +408,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
 #pragma unroll
       for (int r = 0; r < RPL; ++r) m2 = max(m2, (base + 64 * r + lane != lrow) ? key[r] : 0ull);
       m2 = wave_max_key(m2);
-      if (lane == 0) ll_put_double(slot + 2, seq, key_value(m2));
+      if (lane == 0) ll_put_value(slot + 2, seq, static_cast<T>(key_value(m2)));
     }
     DLU_PHASE(1)
 
@@ -396,9 +426,9 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
     if (s_fail) goto leave;
 
     {
-      double wc[RPL];  // private copy of column c+2
+      T wc[RPL];  // private copy of column c+2
 #pragma unroll
-      for (int r = 0; r < RPL; ++r) wc[r] = (c + 2 < width) ? s_win[cb][64 * r + lane] : 0.0;
+      for (int r = 0; r < RPL; ++r) wc[r] = (c + 2 < width) ? s_win[cb][64 * r + lane] : T(0);
 
       // Every wave reduces all candidates itself, with the same rule.
       constexpr int GQ = kMaxPanelGroups / 64;
@@ -443,27 +473,28 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       const uint64_t *wslot = P.slots + (static_cast<size_t>(par) * P.groups + gwin) * SW;
       const uint64_t *wrow = wslot + kSlotHeaderWords;
       const uint64_t *src = nullptr;
-      if (lane < 16 && col0 + lane < width) src = wrow + 2 * (col0 + lane);
-      if (lane >= 16 && lane < 19 && c + lane - 16 < width) src = wrow + 2 * (c + lane - 16);
-      if (p_here && lane >= 19 && lane < 22 && c + lane - 19 < width) src = rk + 2 * (c + lane - 19);
+      if (lane < 16 && col0 + lane < width) src = wrow + WPV * (col0 + lane);
+      if (lane >= 16 && lane < 19 && c + lane - 16 < width) src = wrow + WPV * (c + lane - 16);
+      if (p_here && lane >= 19 && lane < 22 && c + lane - 19 < width)
+        src = rk + WPV * (c + lane - 19);
       if (lane == 22) src = wslot + 2;
       if (p_here && lane >= 32 && lane < 48 && col0 + lane - 32 < width)
-        src = rk + 2 * (col0 + lane - 32);
-      double got = 0.0;
+        src = rk + WPV * (col0 + lane - 32);
+      T got = T(0);
       if (src != nullptr) {
-        unsigned w[2];
-        if (!ll_get<2>(src, seq, w, P, t0)) s_fail = 1;
-        got = join_double(w[0], w[1]);
+        unsigned w[WPV];
+        if (!ll_get<WPV>(src, seq, w, P, t0)) s_fail = 1;
+        got = Words<T>::join(w);
       }
       if (lane < 16) s_u[wave][lane] = got;
       if (lane >= 32 && lane < 48) s_k[wave][lane - 32] = got;
-      const double piv = lane_value(got, 16);
-      const double u1 = lane_value(got, 17), u2 = lane_value(got, 18);
-      const double kc = lane_value(got, 19);
-      const double k1 = lane_value(got, 20), k2 = lane_value(got, 21);
-      const double g2 = fmax(gs, lane_value(got, 22));
+      const T piv = lane_value(got, 16);
+      const T u1 = lane_value(got, 17), u2 = lane_value(got, 18);
+      const T kc = lane_value(got, 19);
+      const T k1 = lane_value(got, 20), k2 = lane_value(got, 21);
+      const double g2 = fmax(gs, static_cast<double>(lane_value(got, 22)));
       wave_sync_lds();
-      double u[16];
+      T u[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) u[j] = s_u[wave][j];
       DLU_PHASE(4)
@@ -479,7 +510,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
       // (column c times the reciprocal of the pivot) and apply the rank-1 update to the panel
       // columns right of c and to the window.  Every row is updated: those at or above k hold
       // values that are never read again, and padding rows beyond m are never written back.
-      const double recip = 1.0 / piv;
+      const T recip = T(1) / piv;
       const int jc = c - col0;  // this wave's column index of c (may be out of 0..15)
 #pragma unroll
       for (int r = 0; r < RPL; ++r) {
@@ -491,7 +522,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams P) {
           wb[r] = k1;
           wc[r] = k2;
         }
-        const double l = (is_p ? kc : wa[r]) * recip;
+        const T l = (is_p ? kc : wa[r]) * recip;
         if (jc < 0) {  // wave-uniform: all of this wave's columns lie right of c
 #pragma unroll
           for (int j = 0; j < 16; ++j) v[r][j] = fma(-l, u[j], v[r][j]);
@@ -639,13 +670,13 @@ leave:
   if (g == 0 && threadIdx.x == 0) P.ctl->stop = P.pb;
 }
 
-template <int NB>
-hipError_t launch_nb(const PanelParams &p, int rows_per_lane, hipStream_t s) {
+template <typename T, int NB>
+hipError_t launch_nb(const PanelParams<T> &p, int rows_per_lane, hipStream_t s) {
   const dim3 grid(p.groups), block(NB * 4);
   switch (rows_per_lane) {
-    case 1: hipLaunchKernelGGL((panel_kernel<NB, 1>), grid, block, 0, s, p); break;
-    case 2: hipLaunchKernelGGL((panel_kernel<NB, 2>), grid, block, 0, s, p); break;
-    case 4: hipLaunchKernelGGL((panel_kernel<NB, 4>), grid, block, 0, s, p); break;
+    case 1: hipLaunchKernelGGL((panel_kernel<T, NB, 1>), grid, block, 0, s, p); break;
+    case 2: hipLaunchKernelGGL((panel_kernel<T, NB, 2>), grid, block, 0, s, p); break;
+    case 4: hipLaunchKernelGGL((panel_kernel<T, NB, 4>), grid, block, 0, s, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -653,14 +684,18 @@ hipError_t launch_nb(const PanelParams &p, int rows_per_lane, hipStream_t s) {
 
 }  // namespace
 
-hipError_t launch_panel(const PanelParams &p, int nb, int rows_per_lane, hipStream_t s) {
+template <typename T>
+hipError_t launch_panel(const PanelParams<T> &p, int nb, int rows_per_lane, hipStream_t s) {
   if (p.groups < 1 || p.groups > kMaxPanelGroups) return hipErrorInvalidValue;
   switch (nb) {
-    case 64: return launch_nb<64>(p, rows_per_lane, s);
-    case 128: return launch_nb<128>(p, rows_per_lane, s);
-    case 256: return launch_nb<256>(p, rows_per_lane, s);
+    case 64: return launch_nb<T, 64>(p, rows_per_lane, s);
+    case 128: return launch_nb<T, 128>(p, rows_per_lane, s);
+    case 256: return launch_nb<T, 256>(p, rows_per_lane, s);
     default: return hipErrorInvalidValue;
   }
 }
+
+template hipError_t launch_panel<float>(const PanelParams<float> &, int, int, hipStream_t);
+template hipError_t launch_panel<double>(const PanelParams<double> &, int, int, hipStream_t);
 
 }  // namespace dlu

@@ -47,7 +47,9 @@ __device__ __forceinline__ void block_best(double &v, unsigned long long &o) {
 }
 
 // One workgroup per group of columns; the last workgroup to finish reduces the partials.
-__global__ __launch_bounds__(256) void active_search_kernel(const double *__restrict__ a, int lda,
+// Magnitudes are compared in double (exact for float entries).
+template <typename T>
+__global__ __launch_bounds__(256) void active_search_kernel(const T *__restrict__ a, int lda,
                                                            int m, int n, int k,
                                                            double *partial_v,
                                                            unsigned long long *partial_o,
@@ -56,10 +58,10 @@ __global__ __launch_bounds__(256) void active_search_kernel(const double *__rest
   double best = 0.0;  // a zero entry never displaces the initial candidate
   unsigned long long bo = 0ull;
   for (int col = blockIdx.x; col < cols; col += gridDim.x) {
-    const double *cp = a + static_cast<size_t>(k) + static_cast<size_t>(k + col) * lda;
+    const T *cp = a + static_cast<size_t>(k) + static_cast<size_t>(k + col) * lda;
     const unsigned long long base = static_cast<unsigned long long>(col) * rows;
     for (int r = threadIdx.x; r < rows; r += 256) {
-      const double x = fabs(cp[r]);
+      const double x = fabs(static_cast<double>(cp[r]));
       const unsigned long long o = base + r;
       if (better(x, o, best, bo)) {
         best = x;
@@ -101,7 +103,8 @@ __global__ __launch_bounds__(256) void active_search_kernel(const double *__rest
 
 // Step k of a full-pivot fallback, part 1: record the decision and swap rows k and best_row over
 // all n columns.  An all-zero active block marks the deferral instead (info = k).
-__global__ __launch_bounds__(256) void fallback_rows_kernel(double *a, int lda, int n, int k,
+template <typename T>
+__global__ __launch_bounds__(256) void fallback_rows_kernel(T *a, int lda, int n, int k,
                                                            int *ipiv, int *jpiv, Control *ctl) {
   const double best = ctl->best_value;
   if (!(best > 0.0)) {
@@ -119,24 +122,25 @@ __global__ __launch_bounds__(256) void fallback_rows_kernel(double *a, int lda, 
   }
   if (r == k) return;
   for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
-    double *x = a + static_cast<size_t>(k) + static_cast<size_t>(j) * lda;
-    double *y = a + static_cast<size_t>(r) + static_cast<size_t>(j) * lda;
-    const double t = *x;
+    T *x = a + static_cast<size_t>(k) + static_cast<size_t>(j) * lda;
+    T *y = a + static_cast<size_t>(r) + static_cast<size_t>(j) * lda;
+    const T t = *x;
     *x = *y;
     *y = t;
   }
 }
 
 // Part 2: swap columns k and best_col over all m rows (after the row swap).
-__global__ __launch_bounds__(256) void fallback_cols_kernel(double *a, int lda, int m, int k,
+template <typename T>
+__global__ __launch_bounds__(256) void fallback_cols_kernel(T *a, int lda, int m, int k,
                                                            const Control *ctl) {
   if (ctl->deferred || !(ctl->best_value > 0.0)) return;
   const int c = ctl->best_col;
   if (c == k) return;
-  double *x = a + static_cast<size_t>(k) * lda;
-  double *y = a + static_cast<size_t>(c) * lda;
+  T *x = a + static_cast<size_t>(k) * lda;
+  T *y = a + static_cast<size_t>(c) * lda;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
-    const double t = x[i];
+    const T t = x[i];
     x[i] = y[i];
     y[i] = t;
   }
@@ -144,8 +148,8 @@ __global__ __launch_bounds__(256) void fallback_cols_kernel(double *a, int lda, 
 
 // Net row movement of a panel applied to columns [c0, c1): one wave per column gathers every
 // source row, then writes every destination row.
-template <int NB>
-__global__ __launch_bounds__(256) void row_interchange_kernel(double *__restrict__ a, int lda,
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void row_interchange_kernel(T *__restrict__ a, int lda,
                                                              int c0, int c1,
                                                              const int *__restrict__ swap_list) {
   constexpr int PER = (2 * NB + 63) / 64;
@@ -153,12 +157,12 @@ __global__ __launch_bounds__(256) void row_interchange_kernel(double *__restrict
   const int lane = threadIdx.x & 63;
   const int col = c0 + blockIdx.x * 4 + (threadIdx.x >> 6);
   const bool active = col < c1;
-  double *cp = a + static_cast<size_t>(active ? col : c0) * lda;
-  double vals[PER];
+  T *cp = a + static_cast<size_t>(active ? col : c0) * lda;
+  T vals[PER];
 #pragma unroll
   for (int s = 0; s < PER; ++s) {
     const int i = lane + 64 * s;
-    vals[s] = (active && i < cnt) ? cp[swap_list[2 * NB + i]] : 0.0;
+    vals[s] = (active && i < cnt) ? cp[swap_list[2 * NB + i]] : T(0);
   }
   __syncthreads();  // every read of this column has completed before any write
 #pragma unroll
@@ -185,39 +189,52 @@ hipError_t launch_init_steps(int *ipiv, int *jpiv, double *margins, int steps, h
   return hipGetLastError();
 }
 
-hipError_t launch_active_search(const double *a, int lda, int m, int n, int k, double *partial_v,
+template <typename T>
+hipError_t launch_active_search(const T *a, int lda, int m, int n, int k, double *partial_v,
                                 unsigned long long *partial_o, Control *ctl, hipStream_t s) {
   const int cols = n - k;
   const int grid = cols < 2048 ? (cols > 0 ? cols : 1) : 2048;
-  hipLaunchKernelGGL(active_search_kernel, dim3(grid), dim3(256), 0, s, a, lda, m, n, k, partial_v,
-                     partial_o, ctl);
+  hipLaunchKernelGGL(active_search_kernel<T>, dim3(grid), dim3(256), 0, s, a, lda, m, n, k,
+                     partial_v, partial_o, ctl);
   return hipGetLastError();
 }
 
-hipError_t launch_fallback_apply(double *a, int lda, int m, int n, int k, int *ipiv, int *jpiv,
+template <typename T>
+hipError_t launch_fallback_apply(T *a, int lda, int m, int n, int k, int *ipiv, int *jpiv,
                                  Control *ctl, hipStream_t s) {
   const int gr = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
-  hipLaunchKernelGGL(fallback_rows_kernel, dim3(gr > 0 ? gr : 1), dim3(256), 0, s, a, lda, n, k,
-                     ipiv, jpiv, ctl);
+  hipLaunchKernelGGL(fallback_rows_kernel<T>, dim3(gr > 0 ? gr : 1), dim3(256), 0, s, a, lda, n,
+                     k, ipiv, jpiv, ctl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int gc = (m + 255) / 256 < 1024 ? (m + 255) / 256 : 1024;
-  hipLaunchKernelGGL(fallback_cols_kernel, dim3(gc > 0 ? gc : 1), dim3(256), 0, s, a, lda, m, k,
-                     ctl);
+  hipLaunchKernelGGL(fallback_cols_kernel<T>, dim3(gc > 0 ? gc : 1), dim3(256), 0, s, a, lda, m,
+                     k, ctl);
   return hipGetLastError();
 }
 
-hipError_t launch_row_interchange(double *a, int lda, int c0, int c1, const int *swap_list,
-                                  int nb, hipStream_t s) {
+template <typename T>
+hipError_t launch_row_interchange(T *a, int lda, int c0, int c1, const int *swap_list, int nb,
+                                  hipStream_t s) {
   if (c1 <= c0) return hipSuccess;
   const dim3 grid((c1 - c0 + 3) / 4), block(256);
   switch (nb) {
-    case 64: hipLaunchKernelGGL(row_interchange_kernel<64>, grid, block, 0, s, a, lda, c0, c1, swap_list); break;
-    case 128: hipLaunchKernelGGL(row_interchange_kernel<128>, grid, block, 0, s, a, lda, c0, c1, swap_list); break;
-    case 256: hipLaunchKernelGGL(row_interchange_kernel<256>, grid, block, 0, s, a, lda, c0, c1, swap_list); break;
+    case 64: hipLaunchKernelGGL((row_interchange_kernel<T, 64>), grid, block, 0, s, a, lda, c0, c1, swap_list); break;
+    case 128: hipLaunchKernelGGL((row_interchange_kernel<T, 128>), grid, block, 0, s, a, lda, c0, c1, swap_list); break;
+    case 256: hipLaunchKernelGGL((row_interchange_kernel<T, 256>), grid, block, 0, s, a, lda, c0, c1, swap_list); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
+
+#define DLU_PIVOT_INSTANTIATE(T)                                                                 \
+  template hipError_t launch_active_search<T>(const T *, int, int, int, int, double *,          \
+                                              unsigned long long *, Control *, hipStream_t);    \
+  template hipError_t launch_fallback_apply<T>(T *, int, int, int, int, int *, int *, Control *, \
+                                               hipStream_t);                                     \
+  template hipError_t launch_row_interchange<T>(T *, int, int, int, const int *, int, hipStream_t);
+DLU_PIVOT_INSTANTIATE(float)
+DLU_PIVOT_INSTANTIATE(double)
+#undef DLU_PIVOT_INSTANTIATE
 
 }  // namespace dlu

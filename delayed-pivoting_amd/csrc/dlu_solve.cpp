@@ -1,0 +1,354 @@
+// Host drivers of the solve side: getrs with the delayed-pivoting factors, the FP64 backward
+// error, the plain FP64 solver and the mixed-precision solver (FP32 factor + FP64 iterative
+// refinement).  None of these has a counterpart in the reference, which factors only; they are
+// the solve and refinement the factorization is used for.
+#include "dlu.h"
+#include "dlu_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+namespace dlu {
+namespace {
+
+constexpr int kSolveBlock = 128;  // rows per diagonal block of the triangular solves
+
+// Device buffers of the solve side, grown on demand and kept per device.
+struct SolveCache {
+  enum { A32, A64, V64A, V64B, V32A, V32B, NORMW, NORMS, PERM, COUNT };
+  void *buf[COUNT] = {};
+  size_t cap[COUNT] = {};
+  hipError_t get(int i, size_t bytes, void **out) {
+    if (bytes == 0) bytes = 16;
+    if (cap[i] < bytes) {
+      if (buf[i]) hipFree(buf[i]);
+      buf[i] = nullptr;
+      cap[i] = 0;
+      hipError_t e = hipMalloc(&buf[i], bytes);
+      if (e != hipSuccess) return e;
+      cap[i] = bytes;
+    }
+    *out = buf[i];
+    return hipSuccess;
+  }
+};
+std::mutex g_solve_lock;
+SolveCache g_cache[64];
+
+// Net row order of P b (swaps in step order) and net order of Q y (swaps in reverse order):
+// (P b)[i] = b[prow[i]],  (Q y)[i] = y[pcol[i]].
+void build_perms(int n, const int *ipiv, const int *jpiv, std::vector<int> &both) {
+  both.resize(2 * static_cast<size_t>(n));
+  int *prow = both.data(), *pcol = both.data() + n;
+  for (int i = 0; i < n; ++i) prow[i] = pcol[i] = i;
+  for (int k = 0; k < n; ++k) std::swap(prow[k], prow[ipiv[k]]);
+  for (int k = n - 1; k >= 0; --k) std::swap(pcol[k], pcol[jpiv[k]]);
+}
+
+bool perms_valid(int n, const int *ipiv, const int *jpiv) {
+  for (int k = 0; k < n; ++k)
+    if (ipiv[k] < k || ipiv[k] >= n || jpiv[k] < k || jpiv[k] >= n) return false;
+  return true;
+}
+
+// X = Q U^{-1} L^{-1} P B, in place in b; work holds n x nrhs elements (leading dimension n).
+template <typename T>
+hipError_t getrs_core(int n, int nrhs, const T *lu, int lda, const int *d_prow, const int *d_pcol,
+                      T *b, int ldb, T *work, hipStream_t s) {
+  hipError_t e = launch_gather_rows<T>(b, ldb, work, n, d_prow, n, nrhs, s);
+  for (int r0 = 0; e == hipSuccess && r0 < n; r0 += kSolveBlock) {
+    const int w = std::min(kSolveBlock, n - r0);
+    e = launch_block_trsv<T>(lu, lda, r0, w, work, n, nrhs, false, s);
+    if (e == hipSuccess && r0 + w < n)
+      e = launch_gemm_sub<T>(work + r0 + w, n,
+                             lu + static_cast<size_t>(r0 + w) + static_cast<size_t>(r0) * lda, lda,
+                             work + r0, n, n - r0 - w, nrhs, w, s);
+  }
+  const int last = ((n - 1) / kSolveBlock) * kSolveBlock;
+  for (int r0 = last; e == hipSuccess && r0 >= 0; r0 -= kSolveBlock) {
+    const int w = std::min(kSolveBlock, n - r0);
+    e = launch_block_trsv<T>(lu, lda, r0, w, work, n, nrhs, true, s);
+    if (e == hipSuccess && r0 > 0)
+      e = launch_gemm_sub<T>(work, n, lu + static_cast<size_t>(r0) * lda, lda, work + r0, n, r0,
+                             nrhs, w, s);
+  }
+  if (e == hipSuccess) e = launch_gather_rows<T>(work, n, b, ldb, d_pcol, n, nrhs, s);
+  return e;
+}
+
+bool solve_args(int n, int nrhs, const void *lu, int lda, const int *ipiv, const int *jpiv,
+                int info, const void *b, int ldb) {
+  if (n < 0 || nrhs < 0 || lda < std::max(1, n) || ldb < std::max(1, n)) return false;
+  if (n == 0 || nrhs == 0) return true;
+  if (lu == nullptr || b == nullptr || ipiv == nullptr || jpiv == nullptr) return false;
+  return info == n && perms_valid(n, ipiv, jpiv);
+}
+
+template <typename T>
+dlu_status getrs_device(int n, int nrhs, const T *d_lu, int lda, const int *ipiv,
+                        const int *jpiv, int info, T *d_b, int ldb, hipStream_t s) {
+  if (!solve_args(n, nrhs, d_lu, lda, ipiv, jpiv, info, d_b, ldb)) return DLU_INVALID_ARGUMENT;
+  if (n == 0 || nrhs == 0) return DLU_SUCCESS;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return DLU_RUNTIME_FAILURE;
+  std::lock_guard<std::mutex> guard(g_solve_lock);
+  SolveCache &c = g_cache[device];
+  std::vector<int> perms;
+  build_perms(n, ipiv, jpiv, perms);
+  void *pw = nullptr, *pp = nullptr;
+  hipError_t e = c.get(sizeof(T) == 4 ? SolveCache::V32A : SolveCache::V64A,
+                       sizeof(T) * static_cast<size_t>(n) * nrhs, &pw);
+  if (e == hipSuccess) e = c.get(SolveCache::PERM, 2 * sizeof(int) * static_cast<size_t>(n), &pp);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(pp, perms.data(), 2 * sizeof(int) * static_cast<size_t>(n),
+                       hipMemcpyHostToDevice, s);
+  const int *d_perm = static_cast<const int *>(pp);
+  if (e == hipSuccess)
+    e = getrs_core<T>(n, nrhs, d_lu, lda, d_perm, d_perm + n, d_b, ldb, static_cast<T *>(pw), s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return to_status(e);
+}
+
+// Per column j of X: ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf).  norms must
+// hold 3*nrhs+1 doubles and r n*nrhs doubles; normw 32*n doubles.  Results land in host bwd.
+hipError_t backward_error(int n, int nrhs, const double *a, int lda, const double *x, int ldx,
+                          const double *b, int ldb, double *r, double *normw, double *norms,
+                          bool have_anorm, double *bwd, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  double *anorm = norms, *xn = norms + 1, *rn = norms + 1 + nrhs, *bn = norms + 1 + 2 * nrhs;
+  if (!have_anorm) {
+    e = hipMemsetAsync(anorm, 0, sizeof(double), s);
+    if (e == hipSuccess) e = launch_norm_inf_matrix(a, lda, n, normw, anorm, s);
+  }
+  if (e == hipSuccess) e = hipMemsetAsync(xn, 0, 3 * sizeof(double) * nrhs, s);
+  if (e == hipSuccess) e = launch_residual(a, lda, x, ldx, b, ldb, r, n, n, nrhs, s);
+  if (e == hipSuccess) e = launch_norm_inf_columns(x, ldx, n, nrhs, xn, s);
+  if (e == hipSuccess) e = launch_norm_inf_columns(r, n, n, nrhs, rn, s);
+  if (e == hipSuccess) e = launch_norm_inf_columns(b, ldb, n, nrhs, bn, s);
+  std::vector<double> h(3 * static_cast<size_t>(nrhs) + 1);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), norms, h.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  for (int j = 0; j < nrhs; ++j) {
+    const double den = h[0] * h[1 + j] + h[1 + 2 * nrhs + j];
+    bwd[j] = den > 0.0 ? h[1 + nrhs + j] / den : (h[1 + nrhs + j] > 0.0 ? 1.0 : 0.0);
+  }
+  return hipSuccess;
+}
+
+struct Timer {
+  hipEvent_t e[4] = {};
+  bool ok = true;
+  Timer() {
+    for (hipEvent_t &x : e) ok = ok && hipEventCreate(&x) == hipSuccess;
+  }
+  ~Timer() {
+    for (hipEvent_t x : e)
+      if (x) hipEventDestroy(x);
+  }
+  void mark(int i, hipStream_t s) {
+    if (ok) hipEventRecord(e[i], s);
+  }
+  double seconds(int i, int j) {
+    float ms = 0.0f;
+    if (!ok || hipEventElapsedTime(&ms, e[i], e[j]) != hipSuccess) return 0.0;
+    return ms * 1e-3;
+  }
+};
+
+bool gesv_args(int n, int nrhs, const void *a, int lda, const void *b, int ldb, const void *x,
+               int ldx) {
+  if (n < 0 || nrhs < 0 || lda < std::max(1, n) || ldb < std::max(1, n) ||
+      ldx < std::max(1, n))
+    return false;
+  return n == 0 || nrhs == 0 || (a && b && x);
+}
+
+// Plain FP64 (use_fp32 = false) or mixed precision (use_fp32 = true): factor, solve, refine.
+dlu_status gesv(int n, int nrhs, const double *d_a, int lda, const double *d_b, int ldb,
+                double *d_x, int ldx, hipStream_t s, const dlu_mixed_options *opt,
+                dlu_mixed_report *rep, double *bwd, bool use_fp32) {
+  if (rep) std::memset(rep, 0, sizeof *rep);
+  if (!gesv_args(n, nrhs, d_a, lda, d_b, ldb, d_x, ldx)) return DLU_INVALID_ARGUMENT;
+  const double tol = (opt && opt->tol > 0.0) ? opt->tol : 1e-13;
+  const int max_iter = (opt && opt->max_iter > 0) ? opt->max_iter : 30;
+  const bool fallback = !(opt && opt->fp64_fallback < 0);
+  dlu_options fopt{};
+  fopt.nb = opt ? opt->nb : 0;
+  if (n == 0 || nrhs == 0) {
+    if (rep) rep->converged = 1;
+    return DLU_SUCCESS;
+  }
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return DLU_RUNTIME_FAILURE;
+  std::lock_guard<std::mutex> guard(g_solve_lock);
+  SolveCache &c = g_cache[device];
+  const size_t nn = static_cast<size_t>(n) * n, nr = static_cast<size_t>(n) * nrhs;
+  void *p_r = nullptr, *p_w64 = nullptr, *p_r32 = nullptr, *p_w32 = nullptr, *p_nw = nullptr,
+       *p_norms = nullptr, *p_perm = nullptr;
+  hipError_t e = c.get(SolveCache::V64A, nr * sizeof(double), &p_r);
+  if (e == hipSuccess) e = c.get(SolveCache::V64B, nr * sizeof(double), &p_w64);
+  if (e == hipSuccess) e = c.get(SolveCache::V32A, nr * sizeof(float), &p_r32);
+  if (e == hipSuccess) e = c.get(SolveCache::V32B, nr * sizeof(float), &p_w32);
+  if (e == hipSuccess) e = c.get(SolveCache::NORMW, 32 * static_cast<size_t>(n) * sizeof(double), &p_nw);
+  if (e == hipSuccess) e = c.get(SolveCache::NORMS, (3 * static_cast<size_t>(nrhs) + 1) * sizeof(double), &p_norms);
+  if (e == hipSuccess) e = c.get(SolveCache::PERM, 2 * sizeof(int) * static_cast<size_t>(n), &p_perm);
+  if (e != hipSuccess) return to_status(e);
+  double *r = static_cast<double *>(p_r), *w64 = static_cast<double *>(p_w64);
+  float *r32 = static_cast<float *>(p_r32), *w32 = static_cast<float *>(p_w32);
+  double *normw = static_cast<double *>(p_nw), *norms = static_cast<double *>(p_norms);
+  int *d_perm = static_cast<int *>(p_perm);
+  std::vector<int> ipiv(n), jpiv(n), perms;
+  std::vector<double> be(nrhs, 1.0);
+  Timer tm;
+  tm.mark(0, s);
+
+  // Factor: FP32 copy of A (mixed) or FP64 copy of A (plain, and the mixed fallback).
+  int info = -1;
+  bool fp32 = use_fp32;
+  void *p_f = nullptr;
+  if (fp32) {
+    if ((e = c.get(SolveCache::A32, nn * sizeof(float), &p_f)) != hipSuccess) return to_status(e);
+    if ((e = launch_to_float(d_a, lda, static_cast<float *>(p_f), n, n, n, s)) != hipSuccess)
+      return to_status(e);
+    const dlu_status st = getrf_device<float>(n, n, static_cast<float *>(p_f), n, ipiv.data(),
+                                              jpiv.data(), &info, s, &fopt, nullptr, nullptr);
+    if (st != DLU_SUCCESS) return st;
+    if (rep) rep->factor_info = info;
+    if (info != n) {
+      if (!fallback) return DLU_INVALID_ARGUMENT;
+      fp32 = false;  // the FP32 factor left a delayed suffix: factor in FP64 instead
+    }
+  }
+  if (!fp32) {
+    if ((e = c.get(SolveCache::A64, nn * sizeof(double), &p_f)) != hipSuccess) return to_status(e);
+    if ((e = hipMemcpy2DAsync(p_f, n * sizeof(double), d_a, lda * sizeof(double),
+                              n * sizeof(double), n, hipMemcpyDeviceToDevice, s)) != hipSuccess)
+      return to_status(e);
+    const dlu_status st = getrf_device<double>(n, n, static_cast<double *>(p_f), n, ipiv.data(),
+                                               jpiv.data(), &info, s, &fopt, nullptr, nullptr);
+    if (st != DLU_SUCCESS) return st;
+    if (rep) {
+      rep->used_fp64 = 1;
+      if (!use_fp32) rep->factor_info = info;
+    }
+    if (info != n) return DLU_INVALID_ARGUMENT;  // singular to working precision: no solve
+  }
+  tm.mark(1, s);
+  build_perms(n, ipiv.data(), jpiv.data(), perms);
+  if ((e = hipMemcpyAsync(d_perm, perms.data(), 2 * sizeof(int) * static_cast<size_t>(n),
+                          hipMemcpyHostToDevice, s)) != hipSuccess)
+    return to_status(e);
+
+  // x = 0, r = b; then x += solve(r), r = b - A x until every column meets the tolerance.
+  if ((e = hipMemset2DAsync(d_x, ldx * sizeof(double), 0, n * sizeof(double), nrhs, s)) !=
+          hipSuccess ||
+      (e = hipMemcpy2DAsync(r, n * sizeof(double), d_b, ldb * sizeof(double), n * sizeof(double),
+                            nrhs, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+      (e = hipMemsetAsync(norms, 0, sizeof(double), s)) != hipSuccess ||
+      (e = launch_norm_inf_matrix(d_a, lda, n, normw, norms, s)) != hipSuccess)
+    return to_status(e);
+  int it = 0;
+  bool converged = false;
+  while (it < max_iter) {
+    ++it;
+    if (fp32) {
+      e = launch_to_float(r, n, r32, n, n, nrhs, s);
+      if (e == hipSuccess)
+        e = getrs_core<float>(n, nrhs, static_cast<const float *>(p_f), n, d_perm, d_perm + n, r32,
+                              n, w32, s);
+      if (e == hipSuccess) e = launch_add_correction<float>(d_x, ldx, r32, n, n, nrhs, s);
+    } else {
+      e = getrs_core<double>(n, nrhs, static_cast<const double *>(p_f), n, d_perm, d_perm + n, r,
+                             n, w64, s);
+      if (e == hipSuccess) e = launch_add_correction<double>(d_x, ldx, r, n, n, nrhs, s);
+    }
+    if (e == hipSuccess)
+      e = backward_error(n, nrhs, d_a, lda, d_x, ldx, d_b, ldb, r, normw, norms, true, be.data(),
+                         s);
+    if (e != hipSuccess) return to_status(e);
+    const double worst = *std::max_element(be.begin(), be.end());
+    if (worst <= tol) {
+      converged = true;
+      break;
+    }
+  }
+  tm.mark(2, s);
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return to_status(e);
+  if (bwd)
+    for (int j = 0; j < nrhs; ++j) bwd[j] = be[j];
+  if (rep) {
+    rep->iterations = it;
+    rep->converged = converged ? 1 : 0;
+    rep->factor_seconds = tm.seconds(0, 1);
+    rep->refine_seconds = tm.seconds(1, 2);
+    rep->total_seconds = tm.seconds(0, 2);
+    rep->max_backward_error = *std::max_element(be.begin(), be.end());
+  }
+  return DLU_SUCCESS;
+}
+
+}  // namespace
+}  // namespace dlu
+
+extern "C" {
+
+dlu_status dlu_getrs_device(int n, int nrhs, const double *d_lu, int lda, const int *ipiv,
+                            const int *jpiv, int info, double *d_b, int ldb, void *stream) {
+  return dlu::getrs_device<double>(n, nrhs, d_lu, lda, ipiv, jpiv, info, d_b, ldb,
+                                   static_cast<hipStream_t>(stream));
+}
+
+dlu_status dlu_sgetrs_device(int n, int nrhs, const float *d_lu, int lda, const int *ipiv,
+                             const int *jpiv, int info, float *d_b, int ldb, void *stream) {
+  return dlu::getrs_device<float>(n, nrhs, d_lu, lda, ipiv, jpiv, info, d_b, ldb,
+                                  static_cast<hipStream_t>(stream));
+}
+
+dlu_status dlu_backward_error_device(int n, int nrhs, const double *d_a, int lda,
+                                     const double *d_x, int ldx, const double *d_b, int ldb,
+                                     double *bwd, void *stream) {
+  if (!dlu::gesv_args(n, nrhs, d_a, lda, d_b, ldb, d_x, ldx) || (nrhs > 0 && n > 0 && !bwd))
+    return DLU_INVALID_ARGUMENT;
+  if (n == 0 || nrhs == 0) {
+    for (int j = 0; j < nrhs; ++j) bwd[j] = 0.0;
+    return DLU_SUCCESS;
+  }
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return DLU_RUNTIME_FAILURE;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> guard(dlu::g_solve_lock);
+  dlu::SolveCache &c = dlu::g_cache[device];
+  void *p_r = nullptr, *p_nw = nullptr, *p_norms = nullptr;
+  hipError_t e = c.get(dlu::SolveCache::V64A, sizeof(double) * static_cast<size_t>(n) * nrhs, &p_r);
+  if (e == hipSuccess) e = c.get(dlu::SolveCache::NORMW, 32 * static_cast<size_t>(n) * sizeof(double), &p_nw);
+  if (e == hipSuccess) e = c.get(dlu::SolveCache::NORMS, (3 * static_cast<size_t>(nrhs) + 1) * sizeof(double), &p_norms);
+  if (e == hipSuccess)
+    e = dlu::backward_error(n, nrhs, d_a, lda, d_x, ldx, d_b, ldb, static_cast<double *>(p_r),
+                            static_cast<double *>(p_nw), static_cast<double *>(p_norms), false,
+                            bwd, s);
+  return dlu::to_status(e);
+}
+
+dlu_status dlu_gesv_device(int n, int nrhs, const double *d_a, int lda, const double *d_b,
+                           int ldb, double *d_x, int ldx, void *stream,
+                           const dlu_mixed_options *options, dlu_mixed_report *report,
+                           double *bwd) {
+  return dlu::gesv(n, nrhs, d_a, lda, d_b, ldb, d_x, ldx, static_cast<hipStream_t>(stream),
+                   options, report, bwd, false);
+}
+
+dlu_status dlu_gesv_mixed_device(int n, int nrhs, const double *d_a, int lda, const double *d_b,
+                                 int ldb, double *d_x, int ldx, void *stream,
+                                 const dlu_mixed_options *options, dlu_mixed_report *report,
+                                 double *bwd) {
+  return dlu::gesv(n, nrhs, d_a, lda, d_b, ldb, d_x, ldx, static_cast<hipStream_t>(stream),
+                   options, report, bwd, true);
+}
+
+}  // extern "C"

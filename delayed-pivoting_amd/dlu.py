@@ -51,6 +51,21 @@ class Report(ctypes.Structure):
         return d
 
 
+class MixedOptions(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("nb", ctypes.c_int),
+                ("fp64_fallback", ctypes.c_int)]
+
+
+class MixedReport(ctypes.Structure):
+    _fields_ = [("factor_info", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("converged", ctypes.c_int), ("used_fp64", ctypes.c_int),
+                ("factor_seconds", ctypes.c_double), ("refine_seconds", ctypes.c_double),
+                ("total_seconds", ctypes.c_double), ("max_backward_error", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 _ip = ctypes.POINTER(ctypes.c_int)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -78,6 +93,25 @@ def lib():
     L.dlu_gemm_sub_device.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 3 + [
         ctypes.c_int, ctypes.c_void_p]
     L.dlu_gemm_sub_device.restype = ctypes.c_int
+    L.dlu_sgetrf_device.argtypes = sig
+    L.dlu_sgetrf_device.restype = ctypes.c_int
+    L.dlu_sgetrf_device_ex.argtypes = sig + [ctypes.POINTER(Options), ctypes.POINTER(Report), _dp]
+    L.dlu_sgetrf_device_ex.restype = ctypes.c_int
+    rs = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, _ip, _ip, ctypes.c_int,
+          ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    for name in ("dlu_getrs_device", "dlu_sgetrs_device"):
+        getattr(L, name).argtypes = rs
+        getattr(L, name).restype = ctypes.c_int
+    L.dlu_backward_error_device.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p]
+    L.dlu_backward_error_device.restype = ctypes.c_int
+    gs = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+          ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+          ctypes.POINTER(MixedOptions), ctypes.POINTER(MixedReport), _dp]
+    for name in ("dlu_gesv_device", "dlu_gesv_mixed_device"):
+        getattr(L, name).argtypes = gs
+        getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
 
@@ -127,6 +161,59 @@ def getrf_device(m: int, n: int, d_a: int, lda: int, stream=None, options: Optio
           "hybrid": L.dlu_getrf_hybrid_device}[variant]
     st = fn(m, n, d_a, lda, ip, jp, ctypes.byref(info), s)
     return Status(st), ipiv[:p], jpiv[:p], info.value, None, None
+
+
+def sgetrf_device(m: int, n: int, d_a: int, lda: int, stream=None,
+                  options: Optional[dict] = None, margins: bool = False):
+    """FP32 factorization of the float32 device matrix at ``d_a``; same returns as getrf_device."""
+    L = lib()
+    p = max(min(m, n), 0)
+    ipiv = np.zeros(max(p, 1), dtype=np.int32)
+    jpiv = np.zeros(max(p, 1), dtype=np.int32)
+    info = ctypes.c_int(-7)
+    s = _stream_ptr(stream)
+    opt = Options(**(options or {}))
+    rep = Report()
+    marg = np.zeros(max(p, 1), dtype=np.float64) if margins else None
+    st = L.dlu_sgetrf_device_ex(m, n, d_a, lda, ipiv.ctypes.data_as(_ip),
+                                jpiv.ctypes.data_as(_ip), ctypes.byref(info), s,
+                                ctypes.byref(opt), ctypes.byref(rep),
+                                marg.ctypes.data_as(_dp) if margins else None)
+    return (Status(st), ipiv[:p], jpiv[:p], info.value, rep.as_dict(),
+            marg[:p] if margins else None)
+
+
+def getrs_device(n: int, nrhs: int, d_lu: int, lda: int, ipiv, jpiv, info: int, d_b: int,
+                 ldb: int, stream=None, single: bool = False) -> Status:
+    """Solve with factors from getrf_device (or sgetrf_device with ``single=True``); d_b := X."""
+    ip = np.ascontiguousarray(ipiv, dtype=np.int32)
+    jp = np.ascontiguousarray(jpiv, dtype=np.int32)
+    fn = lib().dlu_sgetrs_device if single else lib().dlu_getrs_device
+    return Status(fn(n, nrhs, d_lu, lda, ip.ctypes.data_as(_ip), jp.ctypes.data_as(_ip), info,
+                     d_b, ldb, _stream_ptr(stream)))
+
+
+def backward_error_device(n: int, nrhs: int, d_a: int, lda: int, d_x: int, ldx: int, d_b: int,
+                          ldb: int, stream=None):
+    """Per-column ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf): (status, array)."""
+    out = np.zeros(max(nrhs, 1), dtype=np.float64)
+    st = lib().dlu_backward_error_device(n, nrhs, d_a, lda, d_x, ldx, d_b, ldb,
+                                         out.ctypes.data_as(_dp), _stream_ptr(stream))
+    return Status(st), out[:nrhs]
+
+
+def gesv_device(n: int, nrhs: int, d_a: int, lda: int, d_b: int, ldb: int, d_x: int, ldx: int,
+                stream=None, options: Optional[dict] = None, mixed: bool = True):
+    """A X = B (FP64 in and out): mixed precision (FP32 factor + FP64 refinement) or plain FP64.
+
+    Returns (status, report dict, per-column backward errors)."""
+    opt = MixedOptions(**(options or {}))
+    rep = MixedReport()
+    bwd = np.zeros(max(nrhs, 1), dtype=np.float64)
+    fn = lib().dlu_gesv_mixed_device if mixed else lib().dlu_gesv_device
+    st = fn(n, nrhs, d_a, lda, d_b, ldb, d_x, ldx, _stream_ptr(stream), ctypes.byref(opt),
+            ctypes.byref(rep), bwd.ctypes.data_as(_dp))
+    return Status(st), rep.as_dict(), bwd[:nrhs]
 
 
 def getrf_host(a: np.ndarray, variant: str = "host", stream=None):

@@ -39,6 +39,16 @@ def delay_stress(n: int, seed: int, frac_zero: float = 0.05, frac_tiny: float = 
     return a, zero_cols, tiny_cols, off
 
 
+def conditioned(n: int, seed: int, smin: float = 1e-6) -> np.ndarray:
+    """A = U diag(s) V^T with U, V the Q factors of Householder QR of seeded Gaussians and
+    s_i = smin ** (i / (n - 1)) (geometric from 1 to smin): the mixed-precision workload."""
+    rng = np.random.default_rng(seed)
+    u, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    v, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = smin ** (np.arange(n) / max(n - 1, 1))
+    return np.asfortranarray((u * s) @ v.T)
+
+
 def diag_rank(n: int, r: int) -> np.ndarray:
     a = np.zeros((n, n), order="F")
     for k in range(r):

@@ -106,6 +106,65 @@ dlu_status dlu_getrf_device_ex(int m, int n, double *d_a, int lda, int *ipiv, in
                                int *info, void *stream, const dlu_options *options,
                                dlu_report *report, double *margins);
 
+/* The same factorization in IEEE binary32: the FP32 factor of the mixed-precision solver
+ * (dlu_gesv_mixed_device).  Same semantics and pivoting rules as dlu_getrf_device, with every
+ * product, multiplier and update computed in FP32 (FP32 MFMA for the trailing update). */
+dlu_status dlu_sgetrf_device(int m, int n, float *d_a, int lda, int *ipiv, int *jpiv, int *info,
+                             void *stream);
+dlu_status dlu_sgetrf_device_ex(int m, int n, float *d_a, int lda, int *ipiv, int *jpiv,
+                                int *info, void *stream, const dlu_options *options,
+                                dlu_report *report, double *margins);
+
+/* ---- Solve side (no counterpart in the reference, which factors only; its example checks the
+ * factors by reconstruction, delayed_lu/example.cpp:43-126) ------------------------------------ */
+
+/* Solve A X = B with the factors of dlu_getrf_device (P*A*Q = L*U): X = Q U^-1 L^-1 P B, the
+ * row swaps applied to B in step order and the column swaps to the solution in reverse order.
+ * Needs a complete elimination (square, info == n); a delayed suffix has no solve here.
+ * d_b (n x nrhs, leading dimension ldb) is overwritten by X.  ipiv/jpiv are the HOST arrays the
+ * factorization returned.  Synchronizes `stream` before returning. */
+dlu_status dlu_getrs_device(int n, int nrhs, const double *d_lu, int lda, const int *ipiv,
+                            const int *jpiv, int info, double *d_b, int ldb, void *stream);
+dlu_status dlu_sgetrs_device(int n, int nrhs, const float *d_lu, int lda, const int *ipiv,
+                             const int *jpiv, int info, float *d_b, int ldb, void *stream);
+
+/* Per right-hand side j, in FP64 on the device:
+ *   bwd[j] = ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)   (bwd: HOST array). */
+dlu_status dlu_backward_error_device(int n, int nrhs, const double *d_a, int lda,
+                                     const double *d_x, int ldx, const double *d_b, int ldb,
+                                     double *bwd, void *stream);
+
+typedef struct dlu_mixed_options {
+  double tol;        /* target backward error of every column (default 1e-13) */
+  int max_iter;      /* refinement sweeps at most (default 30) */
+  int nb;            /* panel width of the factorization (0: library default) */
+  int fp64_fallback; /* 0 (default): factor in FP64 when the FP32 factor leaves a delayed suffix;
+                        < 0: report DLU_INVALID_ARGUMENT instead */
+} dlu_mixed_options;
+
+typedef struct dlu_mixed_report {
+  int factor_info;   /* info of the (first) factorization */
+  int iterations;    /* solve + residual sweeps */
+  int converged;     /* 1 when every column met tol */
+  int used_fp64;     /* 1 when the factors used were FP64 */
+  double factor_seconds, refine_seconds, total_seconds;  /* event-timed on `stream` */
+  double max_backward_error;
+} dlu_mixed_report;
+
+/* A X = B, FP64 in and out (A untouched).  dlu_gesv_mixed_device factors an FP32 copy of A with
+ * the FP32 delayed LU, then refines in FP64: x += solve32(b - A x) until every column's backward
+ * error (as in dlu_backward_error_device) is at most tol.  dlu_gesv_device factors an FP64 copy
+ * and runs the same loop (normally one sweep).  bwd (HOST, nrhs doubles, may be NULL) receives the
+ * final backward errors.  Synchronous on `stream`. */
+dlu_status dlu_gesv_device(int n, int nrhs, const double *d_a, int lda, const double *d_b,
+                           int ldb, double *d_x, int ldx, void *stream,
+                           const dlu_mixed_options *options, dlu_mixed_report *report,
+                           double *bwd);
+dlu_status dlu_gesv_mixed_device(int n, int nrhs, const double *d_a, int lda, const double *d_b,
+                                 int ldb, double *d_x, int ldx, void *stream,
+                                 const dlu_mixed_options *options, dlu_mixed_report *report,
+                                 double *bwd);
+
 /* Kernel-level entry point of the trailing update: C(MxN) -= A(MxK) * B(KxN), column-major
  * device matrices sharing leading dimension ld.  Asynchronous on `stream`. */
 dlu_status dlu_gemm_sub_device(int M, int N, int K, double *C, const double *A, const double *B,
