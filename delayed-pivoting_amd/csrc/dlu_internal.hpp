@@ -162,7 +162,31 @@ template <typename T>
 hipError_t launch_add_correction(double *x, int ldx, const T *d, int ldd, int n, int ncol,
                                  hipStream_t s);
 
+// ---- Batched small matrices (dlu_batch.hip) ----------------------------------------------------
+constexpr int kBatchMaxRhs = DLU_BATCH_MAX_RHS;
+typedef dlu_batch_item BatchItem;
+// One workgroup per item; d_items is a device array of `count` items, every n <= nmax <= 128.
+hipError_t launch_batch_gesv(const BatchItem *d_items, int count, int nmax, hipStream_t s);
+// a[i] in [-1, 1), deterministic in (seed, i).
+hipError_t launch_fill_uniform(double *a, size_t count, unsigned long long seed, hipStream_t s);
+
 // ---- Host drivers shared between translation units -------------------------------------------
+
+// A factorization kept on the device for later solves (buffers owned by the caller).
+struct DeviceFactor {
+  int n = 0;
+  bool fp32 = false;
+  int info = -1;
+  void *lu = nullptr;     // n x n, leading dimension n, float or double
+  int *d_perm = nullptr;  // net row order of P b, then net order of Q y (n each)
+};
+dlu_status make_factor(int n, const double *d_a, int lda, bool fp32, int nb, float *lu32,
+                       double *lu64, int *d_perm, DeviceFactor &f, int *first_info,
+                       hipStream_t s);
+dlu_status solve_refine(const DeviceFactor &f, int nrhs, const double *d_a, int lda,
+                        const double *d_b, int ldb, double *d_x, int ldx, double tol,
+                        int max_iter, hipStream_t s, int *iterations, double *bwd,
+                        bool *converged);
 dlu_status to_status(hipError_t e);
 template <typename T>
 dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int *info,

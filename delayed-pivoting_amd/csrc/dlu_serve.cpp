@@ -1,0 +1,564 @@
+// The solve service: continuous batching of factor + solve requests on one device.
+//
+// Two workers share the device.  The batch worker packs every queued request with n <= cutoff
+// into one launch of the one-workgroup-per-matrix kernel (dlu_batch.hip) per round; requests
+// that arrive while a round runs wait for the next round.  The full worker takes the larger
+// requests one at a time through the blocked factorization (FP64, or FP32 + FP64 refinement).
+// Memory for factors and per-request buffers comes from the stream-ordered allocator and is
+// admitted against an HBM budget: kept factors stay resident until an admission needs their
+// memory (least recently used first, never while a request is using them).  A worker blocks on
+// memory only while it holds none, so the two workers cannot wait on each other.
+#include "dlu.h"
+#include "dlu_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace dlu {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+struct Resident {
+  long long id = -1;
+  int n = 0;
+  bool batched = false;  // factors of the batch kernel (FP64 LU, device ipiv/jpiv)
+  void *mem = nullptr;
+  size_t bytes = 0;
+  DeviceFactor f;        // full path (lu, net permutations)
+  int info = -1;
+  double *lu = nullptr;  // batched path
+  int *ipiv = nullptr, *jpiv = nullptr;
+  int pins = 0;
+  unsigned long long last_use = 0;
+  hipStream_t stream = nullptr;  // stream of the last use (frees are ordered after it)
+};
+
+struct Job {
+  dlu_request r;
+  double t_submit;
+};
+
+}  // namespace
+
+struct Server {
+  dlu_server_config cfg{};
+  int device = 0;
+  int cutoff = 0;
+  size_t budget = 0;
+  Clock::time_point t0 = Clock::now();
+  std::mutex mu;
+  std::condition_variable cv_small, cv_large, cv_mem, cv_done;
+  std::deque<Job> small_q, large_q;
+  std::vector<dlu_completion> done;
+  std::unordered_map<long long, Resident *> store;
+  size_t used = 0;
+  unsigned long long tick = 0;
+  long long inflight = 0;
+  bool stop = false;
+  hipStream_t s_small = nullptr, s_large = nullptr;
+  BatchItem *h_items = nullptr, *d_items = nullptr;
+  int *h_infos = nullptr, *d_infos = nullptr;
+  std::thread t_small, t_large;
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+  // ---- memory admission -------------------------------------------------------------------
+  void evict_locked(Resident *v) {
+    hipFreeAsync(v->mem, v->stream);
+    used -= v->bytes;
+    store.erase(v->id);
+    delete v;
+  }
+  Resident *lru_locked() {
+    Resident *best = nullptr;
+    for (auto &kv : store)
+      if (kv.second->pins == 0 && (!best || kv.second->last_use < best->last_use)) best = kv.second;
+    return best;
+  }
+  // Returns false when the request can never fit, or (block = false) when it does not fit now.
+  bool reserve(size_t bytes, bool block) {
+    std::unique_lock<std::mutex> lk(mu);
+    if (bytes > budget) return false;
+    while (used + bytes > budget) {
+      if (Resident *v = lru_locked()) {
+        evict_locked(v);
+        continue;
+      }
+      if (!block || stop) return false;
+      cv_mem.wait(lk);
+    }
+    used += bytes;
+    return true;
+  }
+  void release(size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu);
+    used -= bytes;
+    cv_mem.notify_all();
+  }
+  Resident *pin(long long id, bool batched) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = store.find(id);
+    if (it == store.end() || it->second->batched != batched) return nullptr;
+    it->second->pins++;
+    it->second->last_use = ++tick;
+    return it->second;
+  }
+  void unpin(Resident *r, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(mu);
+    r->pins--;
+    r->stream = s;
+    cv_mem.notify_all();
+  }
+  void keep_locked(Resident *r) {
+    auto it = store.find(r->id);
+    if (it != store.end()) {  // an older factor under the same id: replace it when idle
+      if (it->second->pins == 0) {
+        evict_locked(it->second);
+      } else {
+        r->id = -1 - r->id;  // keep the old one; this one is dropped below
+      }
+    }
+    if (r->id < 0) {
+      hipFreeAsync(r->mem, r->stream);
+      used -= r->bytes;
+      delete r;
+      return;
+    }
+    r->last_use = ++tick;
+    store[r->id] = r;
+  }
+  void complete_locked(const dlu_completion &c) {
+    done.push_back(c);
+    --inflight;
+    cv_done.notify_all();
+  }
+
+  // ---- the batch worker ----------------------------------------------------------------------
+  void small_loop() {
+    hipSetDevice(device);
+    const int cap = cfg.max_batch;
+    std::vector<Job> batch;
+    std::vector<Resident *> pinned_res, new_res;
+    std::vector<std::pair<void *, size_t>> temps;
+    for (;;) {
+      batch.clear();
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_small.wait(lk, [&] { return stop || !small_q.empty(); });
+        if (stop && small_q.empty()) return;
+        while (!small_q.empty() && static_cast<int>(batch.size()) < cap) {
+          batch.push_back(small_q.front());
+          small_q.pop_front();
+        }
+      }
+      const double t_start = now();
+      pinned_res.assign(batch.size(), nullptr);
+      new_res.assign(batch.size(), nullptr);
+      temps.clear();
+      size_t count = 0;
+      for (size_t i = 0; i < batch.size(); ++i) {
+        const dlu_request &r = batch[i].r;
+        BatchItem it{};
+        it.n = r.n;
+        it.nrhs = r.nrhs;
+        it.b = r.b;
+        it.ldb = r.ldb;
+        it.x = r.x;
+        it.ldx = r.ldx;
+        it.info = d_infos + i;
+        Resident *res = r.factor_of >= 0 ? pin(r.factor_of, true) : nullptr;
+        if (res) {
+          it.a = res->lu;
+          it.lda = r.n;
+          it.ipiv = res->ipiv;
+          it.jpiv = res->jpiv;
+          it.info_in = res->info;
+          it.solve_only = 1;
+          pinned_res[i] = res;
+        } else {
+          const size_t lub = align256(sizeof(double) * static_cast<size_t>(r.n) * r.n);
+          const size_t bytes = lub + align256(2 * sizeof(int) * static_cast<size_t>(r.n));
+          if (!reserve(bytes, count == 0)) {
+            // no memory now: the rest waits for the next round, in order
+            std::lock_guard<std::mutex> lk(mu);
+            for (size_t j = batch.size(); j-- > i;) small_q.push_front(batch[j]);
+            batch.resize(i);
+            break;
+          }
+          void *mem = nullptr;
+          if (hipMallocAsync(&mem, bytes, s_small) != hipSuccess) {
+            release(bytes);
+            std::lock_guard<std::mutex> lk(mu);
+            for (size_t j = batch.size(); j-- > i;) small_q.push_front(batch[j]);
+            batch.resize(i);
+            break;
+          }
+          it.a = r.a;
+          it.lda = r.lda;
+          it.lu = static_cast<double *>(mem);
+          it.ldlu = r.n;
+          it.ipiv = reinterpret_cast<int *>(static_cast<char *>(mem) + lub);
+          it.jpiv = it.ipiv + r.n;
+          if (r.keep) {
+            Resident *nr = new Resident;
+            nr->id = r.id;
+            nr->n = r.n;
+            nr->batched = true;
+            nr->mem = mem;
+            nr->bytes = bytes;
+            nr->lu = it.lu;
+            nr->ipiv = it.ipiv;
+            nr->jpiv = it.jpiv;
+            nr->stream = s_small;
+            new_res[i] = nr;
+          } else {
+            temps.emplace_back(mem, bytes);
+          }
+        }
+        h_items[i] = it;
+        ++count;
+      }
+      if (count == 0) continue;
+      hipError_t e = hipMemcpyAsync(d_items, h_items, count * sizeof(BatchItem),
+                                    hipMemcpyHostToDevice, s_small);
+      if (e == hipSuccess) e = launch_batch_gesv(d_items, static_cast<int>(count), cutoff, s_small);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(h_infos, d_infos, count * sizeof(int), hipMemcpyDeviceToHost, s_small);
+      if (e == hipSuccess) e = hipStreamSynchronize(s_small);
+      const double t_done = now();
+      for (auto &tm : temps) {
+        hipFreeAsync(tm.first, s_small);
+        release(tm.second);
+      }
+      std::lock_guard<std::mutex> lk(mu);
+      for (size_t i = 0; i < count; ++i) {
+        const dlu_request &r = batch[i].r;
+        dlu_completion c{};
+        c.id = r.id;
+        c.status = e == hipSuccess ? DLU_SUCCESS : to_status(e);
+        c.path = 0;
+        c.t_submit = batch[i].t_submit;
+        c.t_start = t_start;
+        c.t_done = t_done;
+        c.backward_error = -1.0;
+        if (pinned_res[i]) {
+          c.info = pinned_res[i]->info;
+          c.reused = 1;
+          pinned_res[i]->pins--;
+          pinned_res[i]->stream = s_small;
+        } else {
+          c.info = h_infos[i];
+          if (new_res[i]) {
+            new_res[i]->info = c.info;
+            keep_locked(new_res[i]);
+          }
+        }
+        if (c.status == DLU_SUCCESS && c.info != r.n) c.status = DLU_INVALID_ARGUMENT;
+        complete_locked(c);
+      }
+      cv_mem.notify_all();
+    }
+  }
+
+  // ---- the full-factorization worker -------------------------------------------------------
+  void large_loop() {
+    hipSetDevice(device);
+    for (;;) {
+      Job j;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_large.wait(lk, [&] { return stop || !large_q.empty(); });
+        if (stop && large_q.empty()) return;
+        j = large_q.front();
+        large_q.pop_front();
+      }
+      const dlu_request &r = j.r;
+      dlu_completion c{};
+      c.id = r.id;
+      c.t_submit = j.t_submit;
+      c.t_start = now();
+      c.backward_error = -1.0;
+      const int n = r.n;
+      Resident *res = r.factor_of >= 0 ? pin(r.factor_of, false) : nullptr;
+      DeviceFactor f;
+      void *mem = nullptr;
+      size_t bytes = 0;
+      dlu_status st = DLU_SUCCESS;
+      if (res) {
+        f = res->f;
+        c.reused = 1;
+      } else {
+        const size_t nn = static_cast<size_t>(n) * n;
+        const size_t perm = align256(2 * sizeof(int) * static_cast<size_t>(n));
+        bool fp32 = cfg.mixed != 0;
+        for (int attempt = 0; attempt < 2 && st == DLU_SUCCESS; ++attempt) {
+          bytes = perm + align256((fp32 ? sizeof(float) : sizeof(double)) * nn);
+          if (!reserve(bytes, true)) {
+            st = DLU_ALLOCATION_FAILED;
+            break;
+          }
+          if (hipMallocAsync(&mem, bytes, s_large) != hipSuccess) {
+            release(bytes);
+            mem = nullptr;
+            st = DLU_ALLOCATION_FAILED;
+            break;
+          }
+          int *d_perm = static_cast<int *>(mem);
+          void *lu = static_cast<char *>(mem) + perm;
+          int first = -1;
+          st = make_factor(n, r.a, r.lda, fp32, 0, fp32 ? static_cast<float *>(lu) : nullptr,
+                           fp32 ? nullptr : static_cast<double *>(lu), d_perm, f, &first,
+                           s_large);
+          if (st == DLU_SUCCESS && fp32 && f.info != n) {
+            // the FP32 factor left a delayed suffix: factor in FP64 instead
+            hipFreeAsync(mem, s_large);
+            release(bytes);
+            mem = nullptr;
+            fp32 = false;
+            continue;
+          }
+          break;
+        }
+      }
+      c.info = f.info;
+      c.path = f.fp32 ? 2 : 1;
+      if (st == DLU_SUCCESS) {
+        if (f.info != n) {
+          st = DLU_INVALID_ARGUMENT;
+        } else {
+          std::vector<double> bwd(std::max(r.nrhs, 1), 0.0);
+          bool conv = false;
+          st = solve_refine(f, r.nrhs, r.a, r.lda, r.b, r.ldb, r.x, r.ldx, cfg.tol, 30, s_large,
+                            &c.iterations, bwd.data(), &conv);
+          if (st == DLU_SUCCESS) c.backward_error = *std::max_element(bwd.begin(), bwd.end());
+        }
+      }
+      hipStreamSynchronize(s_large);
+      c.t_done = now();
+      c.status = st;
+      Resident *kept = nullptr;
+      if (!res && mem && r.keep && f.info == n) {
+        kept = new Resident;
+        kept->id = r.id;
+        kept->n = n;
+        kept->mem = mem;
+        kept->bytes = bytes;
+        kept->f = f;
+        kept->info = f.info;
+        kept->stream = s_large;
+      } else if (!res && mem) {
+        hipFreeAsync(mem, s_large);
+        release(bytes);
+      }
+      std::lock_guard<std::mutex> lk(mu);
+      if (res) {
+        res->pins--;
+        res->stream = s_large;
+      }
+      if (kept) keep_locked(kept);
+      complete_locked(c);
+      cv_mem.notify_all();
+    }
+  }
+
+  // ---- cutoff tuning ---------------------------------------------------------------------------
+  // Per-matrix cost of the batched kernel (a round of 128 matrices, amortized) against one full
+  // factorization + solve, at n = 32..128: the cutoff is the largest n where batching is cheaper.
+  int tune_cutoff() {
+    const int B = 128;
+    int best = 0;
+    double *a = nullptr, *b = nullptr, *lu = nullptr;
+    int *piv = nullptr;
+    const size_t nmax = DLU_BATCH_MAX_N;
+    if (hipMalloc(&a, sizeof(double) * nmax * nmax * B) != hipSuccess) return 64;
+    hipMalloc(&b, sizeof(double) * nmax * B);
+    hipMalloc(&lu, sizeof(double) * nmax * nmax * B);
+    hipMalloc(&piv, sizeof(int) * (2 * nmax + 1) * B);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    for (int n : {32, 64, 96, 128}) {
+      launch_fill_uniform(a, static_cast<size_t>(n) * n * B, 1234u + n, s_small);
+      launch_fill_uniform(b, static_cast<size_t>(n) * B, 99u + n, s_small);
+      for (int i = 0; i < B; ++i) {
+        BatchItem &it = h_items[i];
+        it = BatchItem{};
+        it.a = a + static_cast<size_t>(i) * n * n;
+        it.lda = n;
+        it.lu = lu + static_cast<size_t>(i) * n * n;
+        it.ldlu = n;
+        it.ipiv = piv + static_cast<size_t>(i) * (2 * n + 1);
+        it.jpiv = it.ipiv + n;
+        it.info = it.jpiv + n;
+        it.n = n;
+        it.nrhs = 1;
+        it.b = b + static_cast<size_t>(i) * n;
+        it.ldb = n;
+        it.x = lu + static_cast<size_t>(i) * n * n;  // scratch: overwrites the factors' copy
+        it.ldx = n;
+      }
+      hipMemcpyAsync(d_items, h_items, B * sizeof(BatchItem), hipMemcpyHostToDevice, s_small);
+      float tb = 0.0f, tl = 0.0f;
+      for (int rep = 0; rep < 2; ++rep) {
+        hipEventRecord(e0, s_small);
+        launch_batch_gesv(d_items, B, DLU_BATCH_MAX_N, s_small);
+        hipEventRecord(e1, s_small);
+        hipEventSynchronize(e1);
+        hipEventElapsedTime(&tb, e0, e1);
+      }
+      DeviceFactor f;
+      for (int rep = 0; rep < 2; ++rep) {
+        const auto c0 = Clock::now();
+        if (make_factor(n, a, n, false, 0, nullptr, lu, piv, f, nullptr, s_small) != DLU_SUCCESS ||
+            f.info != n)
+          break;
+        int it = 0;
+        bool conv = false;
+        double be = 0.0;
+        solve_refine(f, 1, a, n, b, n, lu + static_cast<size_t>(n) * n, n, cfg.tol, 2, s_small,
+                     &it, &be, &conv);
+        hipStreamSynchronize(s_small);
+        tl = static_cast<float>(std::chrono::duration<double, std::milli>(Clock::now() - c0).count());
+      }
+      if (tb / B <= tl) best = n;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    hipFree(lu);
+    hipFree(piv);
+    return best;
+  }
+};
+
+}  // namespace dlu
+
+extern "C" {
+
+struct dlu_server : dlu::Server {};
+
+dlu_server *dlu_server_create(const dlu_server_config *config) {
+  dlu_server *s = new dlu_server;
+  if (config) s->cfg = *config;
+  if (!(s->cfg.hbm_budget_gib > 0.0)) s->cfg.hbm_budget_gib = 32.0;
+  if (!(s->cfg.tol > 0.0)) s->cfg.tol = 1e-13;
+  if (s->cfg.max_batch <= 0) s->cfg.max_batch = 1024;
+  s->budget = static_cast<size_t>(s->cfg.hbm_budget_gib * 1073741824.0);
+  if (hipGetDevice(&s->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->s_small, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->s_large, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc(&s->h_items, sizeof(dlu::BatchItem) * s->cfg.max_batch) != hipSuccess ||
+      hipMalloc(&s->d_items, sizeof(dlu::BatchItem) * s->cfg.max_batch) != hipSuccess ||
+      hipHostMalloc(&s->h_infos, sizeof(int) * s->cfg.max_batch) != hipSuccess ||
+      hipMalloc(&s->d_infos, sizeof(int) * s->cfg.max_batch) != hipSuccess) {
+    delete s;
+    return nullptr;
+  }
+  s->cutoff = s->cfg.batch_cutoff > 0 ? std::min(s->cfg.batch_cutoff, DLU_BATCH_MAX_N)
+                                      : s->tune_cutoff();
+  s->t_small = std::thread([s] { s->small_loop(); });
+  s->t_large = std::thread([s] { s->large_loop(); });
+  return s;
+}
+
+int dlu_server_batch_cutoff(const dlu_server *server) { return server ? server->cutoff : 0; }
+
+dlu_status dlu_server_submit(dlu_server *s, const dlu_request *r) {
+  if (!s || !r || r->n < 1 || r->nrhs < 0 || !r->a || r->lda < r->n ||
+      (r->nrhs > 0 && (!r->b || !r->x || r->ldb < r->n || r->ldx < r->n)))
+    return DLU_INVALID_ARGUMENT;
+  const bool small = r->n <= s->cutoff && r->nrhs <= DLU_BATCH_MAX_RHS;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return DLU_INVALID_ARGUMENT;
+    ++s->inflight;
+    (small ? s->small_q : s->large_q).push_back(dlu::Job{*r, s->now()});
+  }
+  (small ? s->cv_small : s->cv_large).notify_one();
+  return DLU_SUCCESS;
+}
+
+int dlu_server_poll(dlu_server *s, dlu_completion *out, int max) {
+  if (!s || !out || max <= 0) return 0;
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int k = std::min(max, static_cast<int>(s->done.size()));
+  std::copy(s->done.begin(), s->done.begin() + k, out);
+  s->done.erase(s->done.begin(), s->done.begin() + k);
+  return k;
+}
+
+dlu_status dlu_server_drain(dlu_server *s) {
+  if (!s) return DLU_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [s] { return s->inflight == 0; });
+  return DLU_SUCCESS;
+}
+
+double dlu_server_now(const dlu_server *s) { return s ? s->now() : 0.0; }
+
+void dlu_server_destroy(dlu_server *s) {
+  if (!s) return;
+  dlu_server_drain(s);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_small.notify_all();
+  s->cv_large.notify_all();
+  s->cv_mem.notify_all();
+  if (s->t_small.joinable()) s->t_small.join();
+  if (s->t_large.joinable()) s->t_large.join();
+  for (auto &kv : s->store) {
+    hipFreeAsync(kv.second->mem, kv.second->stream);
+    delete kv.second;
+  }
+  s->store.clear();
+  hipStreamSynchronize(s->s_small);
+  hipStreamSynchronize(s->s_large);
+  hipFree(s->d_items);
+  hipFree(s->d_infos);
+  hipHostFree(s->h_items);
+  hipHostFree(s->h_infos);
+  hipStreamDestroy(s->s_small);
+  hipStreamDestroy(s->s_large);
+  delete s;
+}
+
+dlu_status dlu_batch_gesv_device(int count, const dlu_batch_item *items, void *stream) {
+  if (count < 0 || (count > 0 && !items)) return DLU_INVALID_ARGUMENT;
+  int nmax = 0;
+  for (int i = 0; i < count; ++i) {
+    const dlu_batch_item &it = items[i];
+    if (it.n < 1 || it.n > DLU_BATCH_MAX_N || it.nrhs < 0 || it.nrhs > DLU_BATCH_MAX_RHS ||
+        !it.a || it.lda < it.n || !it.ipiv || !it.jpiv ||
+        (!it.solve_only && (!it.lu || it.ldlu < it.n || !it.info)) ||
+        (it.nrhs > 0 && (!it.b || !it.x || it.ldb < it.n || it.ldx < it.n)))
+      return DLU_INVALID_ARGUMENT;
+    nmax = std::max(nmax, it.n);
+  }
+  if (count == 0) return DLU_SUCCESS;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  dlu_batch_item *d = nullptr;
+  hipError_t e = hipMallocAsync(&d, sizeof(dlu_batch_item) * count, s);
+  if (e != hipSuccess) return DLU_ALLOCATION_FAILED;
+  e = hipMemcpyAsync(d, items, sizeof(dlu_batch_item) * count, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = dlu::launch_batch_gesv(d, count, nmax, s);
+  hipFreeAsync(d, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return dlu::to_status(e);
+}
+
+}  // extern "C"

@@ -36,8 +36,8 @@ struct SolveCache {
     return hipSuccess;
   }
 };
-std::mutex g_solve_lock;
-SolveCache g_cache[64];
+// Per host thread (the serving workers and callers each keep their own scratch).
+thread_local SolveCache t_cache[64];
 
 // Net row order of P b (swaps in step order) and net order of Q y (swaps in reverse order):
 // (P b)[i] = b[prow[i]],  (Q y)[i] = y[pcol[i]].
@@ -95,8 +95,7 @@ dlu_status getrs_device(int n, int nrhs, const T *d_lu, int lda, const int *ipiv
   if (n == 0 || nrhs == 0) return DLU_SUCCESS;
   int device = 0;
   if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return DLU_RUNTIME_FAILURE;
-  std::lock_guard<std::mutex> guard(g_solve_lock);
-  SolveCache &c = g_cache[device];
+  SolveCache &c = t_cache[device];
   std::vector<int> perms;
   build_perms(n, ipiv, jpiv, perms);
   void *pw = nullptr, *pp = nullptr;
@@ -169,6 +168,121 @@ bool gesv_args(int n, int nrhs, const void *a, int lda, const void *b, int ldb, 
   return n == 0 || nrhs == 0 || (a && b && x);
 }
 
+}  // namespace
+
+// Factor an n x n FP64 matrix for later solves.  lu32 / lu64 have room for n*n floats / doubles
+// (leading dimension n), d_perm for 2n ints.  fp32: factor a float copy into lu32; when that
+// leaves a delayed suffix and lu64 is given, factor an FP64 copy into lu64 instead.
+dlu_status make_factor(int n, const double *d_a, int lda, bool fp32, int nb, float *lu32,
+                       double *lu64, int *d_perm, DeviceFactor &f, int *first_info,
+                       hipStream_t s) {
+  dlu_options fopt{};
+  fopt.nb = nb;
+  std::vector<int> ipiv(n), jpiv(n), perms;
+  int info = -1;
+  hipError_t e;
+  f = DeviceFactor{};
+  f.n = n;
+  f.d_perm = d_perm;
+  if (fp32) {
+    if ((e = launch_to_float(d_a, lda, lu32, n, n, n, s)) != hipSuccess) return to_status(e);
+    const dlu_status st = getrf_device<float>(n, n, lu32, n, ipiv.data(), jpiv.data(), &info, s,
+                                              &fopt, nullptr, nullptr);
+    if (st != DLU_SUCCESS) return st;
+    if (first_info) *first_info = info;
+    f.fp32 = true;
+    f.lu = lu32;
+    f.info = info;
+    if (info != n && lu64 == nullptr) return DLU_SUCCESS;  // the caller decides what to do
+  }
+  if (!fp32 || info != n) {
+    if (lu64 == nullptr) return DLU_INVALID_ARGUMENT;
+    if ((e = hipMemcpy2DAsync(lu64, n * sizeof(double), d_a, lda * sizeof(double),
+                              n * sizeof(double), n, hipMemcpyDeviceToDevice, s)) != hipSuccess)
+      return to_status(e);
+    const dlu_status st = getrf_device<double>(n, n, lu64, n, ipiv.data(), jpiv.data(), &info, s,
+                                               &fopt, nullptr, nullptr);
+    if (st != DLU_SUCCESS) return st;
+    if (first_info && !fp32) *first_info = info;
+    f.fp32 = false;
+    f.lu = lu64;
+    f.info = info;
+  }
+  if (info == n) {
+    build_perms(n, ipiv.data(), jpiv.data(), perms);
+    if ((e = hipMemcpyAsync(d_perm, perms.data(), 2 * sizeof(int) * static_cast<size_t>(n),
+                            hipMemcpyHostToDevice, s)) != hipSuccess)
+      return to_status(e);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return to_status(e);  // perms is local
+  }
+  return DLU_SUCCESS;
+}
+
+// x := the solution of A x = b, refined in FP64 with the factor f: x = 0, r = b; then
+// x += solve(r) and r = b - A x until every column's backward error is at most tol.  Scratch
+// memory is per host thread.
+dlu_status solve_refine(const DeviceFactor &f, int nrhs, const double *d_a, int lda,
+                        const double *d_b, int ldb, double *d_x, int ldx, double tol,
+                        int max_iter, hipStream_t s, int *iterations, double *bwd,
+                        bool *converged) {
+  const int n = f.n;
+  if (f.info != n) return DLU_INVALID_ARGUMENT;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return DLU_RUNTIME_FAILURE;
+  SolveCache &c = t_cache[device];
+  const size_t nr = static_cast<size_t>(n) * nrhs;
+  void *p_r = nullptr, *p_w = nullptr, *p_r32 = nullptr, *p_nw = nullptr, *p_norms = nullptr;
+  hipError_t e = c.get(SolveCache::V64A, nr * sizeof(double), &p_r);
+  if (e == hipSuccess) e = c.get(SolveCache::V64B, nr * sizeof(double), &p_w);
+  if (e == hipSuccess) e = c.get(SolveCache::V32A, nr * sizeof(float), &p_r32);
+  if (e == hipSuccess) e = c.get(SolveCache::NORMW, 32 * static_cast<size_t>(n) * sizeof(double), &p_nw);
+  if (e == hipSuccess) e = c.get(SolveCache::NORMS, (3 * static_cast<size_t>(nrhs) + 1) * sizeof(double), &p_norms);
+  if (e != hipSuccess) return to_status(e);
+  double *r = static_cast<double *>(p_r), *normw = static_cast<double *>(p_nw);
+  double *norms = static_cast<double *>(p_norms);
+  float *r32 = static_cast<float *>(p_r32);
+  std::vector<double> be(nrhs, 1.0);
+  if ((e = hipMemset2DAsync(d_x, ldx * sizeof(double), 0, n * sizeof(double), nrhs, s)) !=
+          hipSuccess ||
+      (e = hipMemcpy2DAsync(r, n * sizeof(double), d_b, ldb * sizeof(double), n * sizeof(double),
+                            nrhs, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+      (e = hipMemsetAsync(norms, 0, sizeof(double), s)) != hipSuccess ||
+      (e = launch_norm_inf_matrix(d_a, lda, n, normw, norms, s)) != hipSuccess)
+    return to_status(e);
+  int it = 0;
+  bool conv = false;
+  while (it < max_iter) {
+    ++it;
+    if (f.fp32) {
+      // the float work buffer reuses the double one (n*nrhs floats fit in n*nrhs doubles)
+      e = launch_to_float(r, n, r32, n, n, nrhs, s);
+      if (e == hipSuccess)
+        e = getrs_core<float>(n, nrhs, static_cast<const float *>(f.lu), n, f.d_perm,
+                              f.d_perm + n, r32, n, static_cast<float *>(p_w), s);
+      if (e == hipSuccess) e = launch_add_correction<float>(d_x, ldx, r32, n, n, nrhs, s);
+    } else {
+      e = getrs_core<double>(n, nrhs, static_cast<const double *>(f.lu), n, f.d_perm,
+                             f.d_perm + n, r, n, static_cast<double *>(p_w), s);
+      if (e == hipSuccess) e = launch_add_correction<double>(d_x, ldx, r, n, n, nrhs, s);
+    }
+    if (e == hipSuccess)
+      e = backward_error(n, nrhs, d_a, lda, d_x, ldx, d_b, ldb, r, normw, norms, true, be.data(),
+                         s);
+    if (e != hipSuccess) return to_status(e);
+    if (*std::max_element(be.begin(), be.end()) <= tol) {
+      conv = true;
+      break;
+    }
+  }
+  if (iterations) *iterations = it;
+  if (converged) *converged = conv;
+  if (bwd)
+    for (int j = 0; j < nrhs; ++j) bwd[j] = be[j];
+  return DLU_SUCCESS;
+}
+
+namespace {
+
 // Plain FP64 (use_fp32 = false) or mixed precision (use_fp32 = true): factor, solve, refine.
 dlu_status gesv(int n, int nrhs, const double *d_a, int lda, const double *d_b, int ldb,
                 double *d_x, int ldx, hipStream_t s, const dlu_mixed_options *opt,
@@ -178,113 +292,47 @@ dlu_status gesv(int n, int nrhs, const double *d_a, int lda, const double *d_b, 
   const double tol = (opt && opt->tol > 0.0) ? opt->tol : 1e-13;
   const int max_iter = (opt && opt->max_iter > 0) ? opt->max_iter : 30;
   const bool fallback = !(opt && opt->fp64_fallback < 0);
-  dlu_options fopt{};
-  fopt.nb = opt ? opt->nb : 0;
   if (n == 0 || nrhs == 0) {
     if (rep) rep->converged = 1;
     return DLU_SUCCESS;
   }
   int device = 0;
   if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return DLU_RUNTIME_FAILURE;
-  std::lock_guard<std::mutex> guard(g_solve_lock);
-  SolveCache &c = g_cache[device];
-  const size_t nn = static_cast<size_t>(n) * n, nr = static_cast<size_t>(n) * nrhs;
-  void *p_r = nullptr, *p_w64 = nullptr, *p_r32 = nullptr, *p_w32 = nullptr, *p_nw = nullptr,
-       *p_norms = nullptr, *p_perm = nullptr;
-  hipError_t e = c.get(SolveCache::V64A, nr * sizeof(double), &p_r);
-  if (e == hipSuccess) e = c.get(SolveCache::V64B, nr * sizeof(double), &p_w64);
-  if (e == hipSuccess) e = c.get(SolveCache::V32A, nr * sizeof(float), &p_r32);
-  if (e == hipSuccess) e = c.get(SolveCache::V32B, nr * sizeof(float), &p_w32);
-  if (e == hipSuccess) e = c.get(SolveCache::NORMW, 32 * static_cast<size_t>(n) * sizeof(double), &p_nw);
-  if (e == hipSuccess) e = c.get(SolveCache::NORMS, (3 * static_cast<size_t>(nrhs) + 1) * sizeof(double), &p_norms);
-  if (e == hipSuccess) e = c.get(SolveCache::PERM, 2 * sizeof(int) * static_cast<size_t>(n), &p_perm);
+  SolveCache &c = t_cache[device];
+  const size_t nn = static_cast<size_t>(n) * n;
+  void *p32 = nullptr, *p64 = nullptr, *pperm = nullptr;
+  hipError_t e = c.get(SolveCache::PERM, 2 * sizeof(int) * static_cast<size_t>(n), &pperm);
+  if (e == hipSuccess && use_fp32) e = c.get(SolveCache::A32, nn * sizeof(float), &p32);
+  if (e == hipSuccess && (!use_fp32 || fallback))
+    e = c.get(SolveCache::A64, nn * sizeof(double), &p64);
   if (e != hipSuccess) return to_status(e);
-  double *r = static_cast<double *>(p_r), *w64 = static_cast<double *>(p_w64);
-  float *r32 = static_cast<float *>(p_r32), *w32 = static_cast<float *>(p_w32);
-  double *normw = static_cast<double *>(p_nw), *norms = static_cast<double *>(p_norms);
-  int *d_perm = static_cast<int *>(p_perm);
-  std::vector<int> ipiv(n), jpiv(n), perms;
-  std::vector<double> be(nrhs, 1.0);
   Timer tm;
   tm.mark(0, s);
-
-  // Factor: FP32 copy of A (mixed) or FP64 copy of A (plain, and the mixed fallback).
-  int info = -1;
-  bool fp32 = use_fp32;
-  void *p_f = nullptr;
-  if (fp32) {
-    if ((e = c.get(SolveCache::A32, nn * sizeof(float), &p_f)) != hipSuccess) return to_status(e);
-    if ((e = launch_to_float(d_a, lda, static_cast<float *>(p_f), n, n, n, s)) != hipSuccess)
-      return to_status(e);
-    const dlu_status st = getrf_device<float>(n, n, static_cast<float *>(p_f), n, ipiv.data(),
-                                              jpiv.data(), &info, s, &fopt, nullptr, nullptr);
-    if (st != DLU_SUCCESS) return st;
-    if (rep) rep->factor_info = info;
-    if (info != n) {
-      if (!fallback) return DLU_INVALID_ARGUMENT;
-      fp32 = false;  // the FP32 factor left a delayed suffix: factor in FP64 instead
-    }
+  DeviceFactor f;
+  int first_info = -1;
+  dlu_status st = make_factor(n, d_a, lda, use_fp32, opt ? opt->nb : 0,
+                              static_cast<float *>(p32), static_cast<double *>(p64),
+                              static_cast<int *>(pperm), f, &first_info, s);
+  if (st != DLU_SUCCESS) return st;
+  if (rep) {
+    rep->factor_info = first_info;
+    rep->used_fp64 = f.fp32 ? 0 : 1;
   }
-  if (!fp32) {
-    if ((e = c.get(SolveCache::A64, nn * sizeof(double), &p_f)) != hipSuccess) return to_status(e);
-    if ((e = hipMemcpy2DAsync(p_f, n * sizeof(double), d_a, lda * sizeof(double),
-                              n * sizeof(double), n, hipMemcpyDeviceToDevice, s)) != hipSuccess)
-      return to_status(e);
-    const dlu_status st = getrf_device<double>(n, n, static_cast<double *>(p_f), n, ipiv.data(),
-                                               jpiv.data(), &info, s, &fopt, nullptr, nullptr);
-    if (st != DLU_SUCCESS) return st;
-    if (rep) {
-      rep->used_fp64 = 1;
-      if (!use_fp32) rep->factor_info = info;
-    }
-    if (info != n) return DLU_INVALID_ARGUMENT;  // singular to working precision: no solve
-  }
+  if (f.info != n) return DLU_INVALID_ARGUMENT;  // singular to working precision: no solve
   tm.mark(1, s);
-  build_perms(n, ipiv.data(), jpiv.data(), perms);
-  if ((e = hipMemcpyAsync(d_perm, perms.data(), 2 * sizeof(int) * static_cast<size_t>(n),
-                          hipMemcpyHostToDevice, s)) != hipSuccess)
-    return to_status(e);
-
-  // x = 0, r = b; then x += solve(r), r = b - A x until every column meets the tolerance.
-  if ((e = hipMemset2DAsync(d_x, ldx * sizeof(double), 0, n * sizeof(double), nrhs, s)) !=
-          hipSuccess ||
-      (e = hipMemcpy2DAsync(r, n * sizeof(double), d_b, ldb * sizeof(double), n * sizeof(double),
-                            nrhs, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
-      (e = hipMemsetAsync(norms, 0, sizeof(double), s)) != hipSuccess ||
-      (e = launch_norm_inf_matrix(d_a, lda, n, normw, norms, s)) != hipSuccess)
-    return to_status(e);
   int it = 0;
-  bool converged = false;
-  while (it < max_iter) {
-    ++it;
-    if (fp32) {
-      e = launch_to_float(r, n, r32, n, n, nrhs, s);
-      if (e == hipSuccess)
-        e = getrs_core<float>(n, nrhs, static_cast<const float *>(p_f), n, d_perm, d_perm + n, r32,
-                              n, w32, s);
-      if (e == hipSuccess) e = launch_add_correction<float>(d_x, ldx, r32, n, n, nrhs, s);
-    } else {
-      e = getrs_core<double>(n, nrhs, static_cast<const double *>(p_f), n, d_perm, d_perm + n, r,
-                             n, w64, s);
-      if (e == hipSuccess) e = launch_add_correction<double>(d_x, ldx, r, n, n, nrhs, s);
-    }
-    if (e == hipSuccess)
-      e = backward_error(n, nrhs, d_a, lda, d_x, ldx, d_b, ldb, r, normw, norms, true, be.data(),
-                         s);
-    if (e != hipSuccess) return to_status(e);
-    const double worst = *std::max_element(be.begin(), be.end());
-    if (worst <= tol) {
-      converged = true;
-      break;
-    }
-  }
+  bool conv = false;
+  std::vector<double> be(nrhs, 1.0);
+  st = solve_refine(f, nrhs, d_a, lda, d_b, ldb, d_x, ldx, tol, max_iter, s, &it, be.data(),
+                    &conv);
+  if (st != DLU_SUCCESS) return st;
   tm.mark(2, s);
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return to_status(e);
   if (bwd)
     for (int j = 0; j < nrhs; ++j) bwd[j] = be[j];
   if (rep) {
     rep->iterations = it;
-    rep->converged = converged ? 1 : 0;
+    rep->converged = conv ? 1 : 0;
     rep->factor_seconds = tm.seconds(0, 1);
     rep->refine_seconds = tm.seconds(1, 2);
     rep->total_seconds = tm.seconds(0, 2);
@@ -322,8 +370,7 @@ dlu_status dlu_backward_error_device(int n, int nrhs, const double *d_a, int lda
   int device = 0;
   if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return DLU_RUNTIME_FAILURE;
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  std::lock_guard<std::mutex> guard(dlu::g_solve_lock);
-  dlu::SolveCache &c = dlu::g_cache[device];
+  dlu::SolveCache &c = dlu::t_cache[device];
   void *p_r = nullptr, *p_nw = nullptr, *p_norms = nullptr;
   hipError_t e = c.get(dlu::SolveCache::V64A, sizeof(double) * static_cast<size_t>(n) * nrhs, &p_r);
   if (e == hipSuccess) e = c.get(dlu::SolveCache::NORMW, 32 * static_cast<size_t>(n) * sizeof(double), &p_nw);

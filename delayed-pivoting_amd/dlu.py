@@ -66,6 +66,36 @@ class MixedReport(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_void_p), ("lda", ctypes.c_int), ("lu", ctypes.c_void_p),
+                ("ldlu", ctypes.c_int), ("ipiv", ctypes.c_void_p), ("jpiv", ctypes.c_void_p),
+                ("info", ctypes.c_void_p), ("info_in", ctypes.c_int), ("n", ctypes.c_int),
+                ("nrhs", ctypes.c_int), ("b", ctypes.c_void_p), ("ldb", ctypes.c_int),
+                ("x", ctypes.c_void_p), ("ldx", ctypes.c_int), ("solve_only", ctypes.c_int)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_gib", ctypes.c_double), ("batch_cutoff", ctypes.c_int),
+                ("mixed", ctypes.c_int), ("tol", ctypes.c_double), ("max_batch", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_longlong), ("n", ctypes.c_int), ("nrhs", ctypes.c_int),
+                ("a", ctypes.c_void_p), ("lda", ctypes.c_int), ("b", ctypes.c_void_p),
+                ("ldb", ctypes.c_int), ("x", ctypes.c_void_p), ("ldx", ctypes.c_int),
+                ("factor_of", ctypes.c_longlong), ("keep", ctypes.c_int)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_longlong), ("status", ctypes.c_int), ("info", ctypes.c_int),
+                ("path", ctypes.c_int), ("iterations", ctypes.c_int), ("reused", ctypes.c_int),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("backward_error", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 _ip = ctypes.POINTER(ctypes.c_int)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -112,8 +142,70 @@ def lib():
     for name in ("dlu_gesv_device", "dlu_gesv_mixed_device"):
         getattr(L, name).argtypes = gs
         getattr(L, name).restype = ctypes.c_int
+    L.dlu_batch_gesv_device.argtypes = [ctypes.c_int, ctypes.POINTER(BatchItem), ctypes.c_void_p]
+    L.dlu_batch_gesv_device.restype = ctypes.c_int
+    L.dlu_server_create.argtypes = [ctypes.POINTER(ServerConfig)]
+    L.dlu_server_create.restype = ctypes.c_void_p
+    L.dlu_server_batch_cutoff.argtypes = [ctypes.c_void_p]
+    L.dlu_server_batch_cutoff.restype = ctypes.c_int
+    L.dlu_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request)]
+    L.dlu_server_submit.restype = ctypes.c_int
+    L.dlu_server_poll.argtypes = [ctypes.c_void_p, ctypes.POINTER(Completion), ctypes.c_int]
+    L.dlu_server_poll.restype = ctypes.c_int
+    L.dlu_server_drain.argtypes = [ctypes.c_void_p]
+    L.dlu_server_drain.restype = ctypes.c_int
+    L.dlu_server_now.argtypes = [ctypes.c_void_p]
+    L.dlu_server_now.restype = ctypes.c_double
+    L.dlu_server_destroy.argtypes = [ctypes.c_void_p]
+    L.dlu_server_destroy.restype = None
     _lib = L
     return L
+
+
+def batch_gesv_device(items, stream=None) -> Status:
+    """One batched launch over a list of BatchItem (host structures holding device pointers)."""
+    arr = (BatchItem * max(len(items), 1))(*items)
+    return Status(lib().dlu_batch_gesv_device(len(items), arr, _stream_ptr(stream)))
+
+
+class Server:
+    """Thin wrapper of dlu_server_*: submit requests, poll completions, drain, close."""
+
+    def __init__(self, budget_gib=32.0, batch_cutoff=0, mixed=False, tol=1e-13, max_batch=1024):
+        cfg = ServerConfig(budget_gib, batch_cutoff, 1 if mixed else 0, tol, max_batch)
+        self._h = lib().dlu_server_create(ctypes.byref(cfg))
+        if not self._h:
+            raise RuntimeError("dlu_server_create failed")
+
+    @property
+    def cutoff(self) -> int:
+        return lib().dlu_server_batch_cutoff(self._h)
+
+    def now(self) -> float:
+        return lib().dlu_server_now(self._h)
+
+    def submit(self, rid, n, nrhs, a, lda, b, ldb, x, ldx, factor_of=-1, keep=False) -> Status:
+        req = Request(rid, n, nrhs, a, lda, b, ldb, x, ldx, factor_of, 1 if keep else 0)
+        return Status(lib().dlu_server_submit(self._h, ctypes.byref(req)))
+
+    def poll(self, max_items=4096):
+        buf = (Completion * max_items)()
+        k = lib().dlu_server_poll(self._h, buf, max_items)
+        return [buf[i].as_dict() for i in range(k)]
+
+    def drain(self) -> Status:
+        return Status(lib().dlu_server_drain(self._h))
+
+    def close(self):
+        if self._h:
+            lib().dlu_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def status_string(status: int) -> str:

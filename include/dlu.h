@@ -165,6 +165,92 @@ dlu_status dlu_gesv_mixed_device(int n, int nrhs, const double *d_a, int lda, co
                                  const dlu_mixed_options *options, dlu_mixed_report *report,
                                  double *bwd);
 
+/* ---- Batched small matrices and serving (no counterpart in the reference) ------------------ */
+
+#define DLU_BATCH_MAX_N 128  /* largest n of the one-workgroup-per-matrix kernel */
+#define DLU_BATCH_MAX_RHS 8
+
+/* One matrix of a batch.  Factor mode: factor A (same pivoting rules as dlu_getrf_device),
+ * write the factors, pivots and info, and when info == n solve for the right-hand sides.
+ * Solve-only mode: `a` holds factors from an earlier factor-mode item (ipiv/jpiv/info_in
+ * likewise) and only the solve runs.  All pointers are device memory; lu may alias a and x may
+ * alias b. */
+typedef struct dlu_batch_item {
+  const double *a;
+  int lda;
+  double *lu;
+  int ldlu;
+  int *ipiv;
+  int *jpiv;
+  int *info;
+  int info_in;
+  int n;
+  int nrhs;
+  const double *b;
+  int ldb;
+  double *x;
+  int ldx;
+  int solve_only;
+} dlu_batch_item;
+
+/* One launch, one workgroup per item (n <= DLU_BATCH_MAX_N, nrhs <= DLU_BATCH_MAX_RHS); the
+ * item array is HOST memory.  Synchronous on `stream`.  Each item's results do not depend on
+ * the other items of the batch. */
+dlu_status dlu_batch_gesv_device(int count, const dlu_batch_item *items, void *stream);
+
+/* A solve service on one device with continuous batching.  Requests (factor + solve of an
+ * n x n FP64 system, or a solve with the resident factor of an earlier request) are admitted as
+ * memory allows; those with n <= the batch cutoff are packed into one batched launch per round
+ * (dlu_batch_gesv_device), while the others run through the full factorization on a second
+ * worker, so both kinds overlap on the GPU.  Factors asked to be kept stay resident until the
+ * HBM budget needs their memory (least recently used first).  Results do not depend on which
+ * other requests were in flight. */
+typedef struct dlu_server dlu_server;
+
+typedef struct dlu_server_config {
+  double hbm_budget_gib; /* resident factors + per-request buffers (default 32) */
+  int batch_cutoff;      /* largest n for the batched kernel, <= DLU_BATCH_MAX_N (0: tuned at
+                            creation by timing both paths) */
+  int mixed;             /* 1: FP32 factor + FP64 refinement above the cutoff; 0: FP64 */
+  double tol;            /* refinement target backward error (default 1e-13) */
+  int max_batch;         /* most requests per batched launch (default 1024) */
+} dlu_server_config;
+
+typedef struct dlu_request {
+  long long id;        /* caller's id, unique among live requests */
+  int n, nrhs;         /* nrhs <= DLU_BATCH_MAX_RHS for batched requests */
+  const double *a;     /* device, the matrix (read until completion; also for reuse requests) */
+  int lda;
+  const double *b;     /* device right-hand sides */
+  int ldb;
+  double *x;           /* device solution */
+  int ldx;
+  long long factor_of; /* -1: factor a; else the id of an earlier request whose kept factor to
+                          reuse (factored again from a if it was evicted) */
+  int keep;            /* 1: keep this request's factor resident */
+} dlu_request;
+
+typedef struct dlu_completion {
+  long long id;
+  int status;            /* dlu_status */
+  int info;              /* of the factorization used */
+  int path;              /* 0 batched, 1 full FP64, 2 full FP32 + refinement */
+  int iterations;        /* refinement sweeps (full paths) */
+  int reused;            /* 1 when a resident factor served the request */
+  double t_submit, t_start, t_done;  /* seconds on the server clock (dlu_server_now) */
+  double backward_error; /* largest over the columns (full paths; -1 when not computed) */
+} dlu_completion;
+
+dlu_server *dlu_server_create(const dlu_server_config *config);
+int dlu_server_batch_cutoff(const dlu_server *server);
+dlu_status dlu_server_submit(dlu_server *server, const dlu_request *request);
+/* Moves up to `max` completions into `out`; returns how many. */
+int dlu_server_poll(dlu_server *server, dlu_completion *out, int max);
+/* Blocks until every submitted request has completed. */
+dlu_status dlu_server_drain(dlu_server *server);
+double dlu_server_now(const dlu_server *server);
+void dlu_server_destroy(dlu_server *server);
+
 /* Kernel-level entry point of the trailing update: C(MxN) -= A(MxK) * B(KxN), column-major
  * device matrices sharing leading dimension ld.  Asynchronous on `stream`. */
 dlu_status dlu_gemm_sub_device(int M, int N, int K, double *C, const double *A, const double *B,
