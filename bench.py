@@ -65,6 +65,166 @@ def cpu_baseline(n_sample, threads):
     }
 
 
+def _timed(torch, fn, reps=2):
+    """Best of `reps` after one warm-up, host wall clock around a synchronized call."""
+    best = None
+    for i in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        if i > 0 and (best is None or t < best[0]):
+            best = (t, res)
+    return best
+
+
+def bench_delay_stress(dlu, torch, dev, n):
+    """Delay-stress FP64: 5% zero columns, 5% scaled by 1e-14, a rank-32 64x64 diagonal block."""
+    import numpy as np
+    workloads = _load("dlu_workloads", os.path.join(PKG, "workloads.py"))
+    log(f"delay-stress n={n}")
+    a, zero_cols, _, _ = workloads.delay_stress(n, n)
+    orig = torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
+    work = torch.empty_like(orig)
+
+    def run():
+        work.copy_(orig)
+        return dlu.getrf_device(n, n, work.data_ptr(), n, options={"nb": 0})
+
+    t, (st, ipiv, jpiv, info, rep, _) = _timed(torch, run)
+    delayed = np.sort(dlu.net_order(jpiv, info, n)[info:])
+    return {"workload": f"delay-stress FP64 n={n} (5% zero cols, 5% x1e-14, rank-32 64x64 block)",
+            "ms": round(t * 1e3, 2), "tflops_2n3_3": round(2.0 * n ** 3 / 3 / t / 1e12, 3),
+            "info": info, "expected_info": n - len(zero_cols),
+            "delayed_set_equals_zero_columns": bool(info == n - len(zero_cols) and
+                                                    np.array_equal(delayed, zero_cols)),
+            "fallbacks": rep["fallbacks"], "panels": rep["panels"], "status": int(st),
+            "time_includes": "device copy of the input (ms-scale) + factorization"}
+
+
+def bench_mixed(dlu, torch, dev, n, nrhs):
+    """FP32 factor + FP64 refinement vs plain FP64 on A = U diag(s) V^T, s from 1 to 1e-6."""
+    log(f"mixed precision n={n}, {nrhs} RHS")
+    g = torch.Generator(device=dev)
+    g.manual_seed(n)
+    u = torch.linalg.qr(torch.randn((n, n), dtype=torch.float64, device=dev, generator=g))[0]
+    v = torch.linalg.qr(torch.randn((n, n), dtype=torch.float64, device=dev, generator=g))[0]
+    s = torch.tensor(1e-6, dtype=torch.float64, device=dev) ** (
+        torch.arange(n, dtype=torch.float64, device=dev) / (n - 1))
+    at = ((u * s) @ v.T).T.contiguous()  # storage rows = columns of A
+    del u, v
+    b = torch.randn((nrhs, n), dtype=torch.float64, device=dev, generator=g)
+    x = torch.zeros_like(b)
+    out = {"workload": f"A = U diag(s) V^T, n={n}, cond 1e6, {nrhs} RHS N(0,1)", "tol": 1e-13}
+    for mixed in (False, True):
+        t, (st, rep, bwd) = _timed(torch, lambda: dlu.gesv_device(
+            n, nrhs, at.data_ptr(), n, b.data_ptr(), n, x.data_ptr(), n, mixed=mixed))
+        key = "mixed" if mixed else "fp64"
+        out[key] = {"ms": round(t * 1e3, 2), "factor_ms": round(rep["factor_seconds"] * 1e3, 2),
+                    "refine_ms": round(rep["refine_seconds"] * 1e3, 2),
+                    "iterations": rep["iterations"], "converged": bool(rep["converged"]),
+                    "max_backward_error": float(bwd.max()), "status": int(st)}
+    out["speedup_mixed_over_fp64"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    del at, b, x
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_serving(dlu, torch, dev, count, seed=20000):
+    """Poisson arrivals of factor + solve requests, n log-uniform in [64, 4096], 1-8 RHS,
+    30% reusing an earlier request's resident factor; FP64 and mixed-precision services."""
+    import math
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        if i >= 8 and rng.random() < 0.3:
+            j = int(rng.integers(0, i))
+            while reqs[j]["reuse"] >= 0:
+                j = reqs[j]["reuse"]
+            reqs.append({"n": reqs[j]["n"], "reuse": j, "nrhs": int(rng.integers(1, 9))})
+        else:
+            n = int(round(math.exp(rng.uniform(math.log(64), math.log(4096)))))
+            reqs.append({"n": n, "reuse": -1, "nrhs": int(rng.integers(1, 9))})
+    log(f"serving: {count} requests, generating inputs on the device")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    mats = {}
+    for i, r in enumerate(reqs):
+        if r["reuse"] < 0:
+            mats[i] = torch.randn((r["n"], r["n"]), dtype=torch.float64, device=dev, generator=g)
+        r["b"] = torch.randn((r["nrhs"], r["n"]), dtype=torch.float64, device=dev, generator=g)
+        r["x"] = torch.zeros_like(r["b"])
+    torch.cuda.synchronize()
+    flops = sum((0 if r["reuse"] >= 0 else 2.0 * r["n"] ** 3 / 3) + 2.0 * r["n"] ** 2 * r["nrhs"]
+                for r in reqs)
+
+    def a_of(i):
+        return mats[reqs[i]["reuse"] if reqs[i]["reuse"] >= 0 else i]
+
+    def run(srv, arrivals):
+        t0 = srv.now()
+        for i, r in enumerate(reqs):
+            if arrivals is not None:
+                while srv.now() - t0 < arrivals[i]:
+                    time.sleep(min(0.0005, max(0.0, arrivals[i] - (srv.now() - t0))))
+            n = r["n"]
+            st = srv.submit(i, n, r["nrhs"], a_of(i).data_ptr(), n, r["b"].data_ptr(), n,
+                            r["x"].data_ptr(), n, factor_of=r["reuse"], keep=r["reuse"] < 0)
+            if st != dlu.Status.SUCCESS:
+                raise RuntimeError(f"submit {i}: {dlu.status_string(st)}")
+        srv.drain()
+        comps = {c["id"]: c for c in srv.poll(count + 16)}
+        return t0, comps
+
+    out = {"workload": f"{count} requests (the first {count} of the seeded stream; the full "
+                       "configuration has 20000), n log-uniform [64,4096], 1-8 RHS, 30% reuse",
+           "flops": flops}
+    srv = dlu.Server(budget_gib=64.0, mixed=False)
+    cutoff = srv.cutoff
+    # capacity of the FP64 service, closed loop over the whole stream
+    t0, comps = run(srv, None)
+    span = max(c["t_done"] for c in comps.values()) - t0
+    capacity = count / span
+    rate = 0.7 * capacity
+    srv.close()
+    out.update({"batch_cutoff": cutoff, "closed_loop_requests_per_s": round(capacity, 1),
+                "arrival_rate_per_s": round(rate, 1)})
+    arrivals = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, count))
+    for mixed in (False, True):
+        for r in reqs:
+            r["x"].zero_()
+        torch.cuda.synchronize()
+        srv = dlu.Server(budget_gib=64.0, mixed=mixed, batch_cutoff=cutoff)
+        t0, comps = run(srv, arrivals)
+        srv.close()
+        lat = np.array([comps[i]["t_done"] - (t0 + arrivals[i]) for i in range(count)])
+        ok = all(comps[i]["status"] == 0 for i in range(count))
+        worst = 0.0
+        for i in range(0, count, max(1, count // 200)):  # a sample, checked on the device
+            n = reqs[i]["n"]
+            st, bw = dlu.backward_error_device(n, reqs[i]["nrhs"], a_of(i).data_ptr(), n,
+                                               reqs[i]["x"].data_ptr(), n,
+                                               reqs[i]["b"].data_ptr(), n)
+            worst = max(worst, float(bw.max()))
+        span = max(c["t_done"] for c in comps.values()) - t0
+        out["mixed" if mixed else "fp64"] = {
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "mean_ms": round(float(lat.mean()) * 1e3, 3),
+            "solves_per_s": round(count / span, 1),
+            "tflops": round(flops / span / 1e12, 3),
+            "batched": sum(1 for c in comps.values() if c["path"] == 0),
+            "reused": sum(1 for c in comps.values() if c["reused"]),
+            "all_succeeded": ok, "max_backward_error_sampled": worst}
+        log(f"serving mixed={mixed}: p50 {out['mixed' if mixed else 'fp64']['p50_ms']} ms "
+            f"p99 {out['mixed' if mixed else 'fp64']['p99_ms']} ms")
+    del mats, reqs
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +240,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verify", type=int, default=0,
                     help="check permutations against the oracle on the full matrix")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the delay-stress, mixed-precision and serving measurements")
+    ap.add_argument("--mixed-n", type=int, default=16384)
+    ap.add_argument("--serve-requests", type=int, default=1500,
+                    help="requests of the serving stream measured per precision mode")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -176,6 +341,14 @@ def main():
         log(f"cpu baseline: oracle on {args.cpu_sample}^2 with {threads} threads")
         cpu = cpu_baseline(args.cpu_sample, threads)
 
+    del orig, work
+    torch.cuda.empty_cache()
+    c3 = c4 = c5 = None
+    if not args.no_extra:
+        c3 = bench_delay_stress(dlu, torch, dev, 8192)
+        c4 = bench_mixed(dlu, torch, dev, args.mixed_n, 16)
+        c5 = bench_serving(dlu, torch, dev, args.serve_requests)
+
     out = {
         "metric": "FP64 LU TFLOP/s @n=16384 (% peak); FP32+IR speedup @bwd err<=1e-13; p99 ms",
         "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
@@ -189,6 +362,7 @@ def main():
         "roofline": roofline, "phases": phases, "cpu_baseline": cpu, "parity": parity,
         "pivot_report": {k: first[2][k] for k in ("fallbacks", "near_ties", "exact_ties",
                                                   "min_margin", "panels")} if first[2] else None,
+        "c3_delay_stress": c3, "c4_mixed_precision": c4, "c5_serving": c5,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -162,6 +162,15 @@ def lib():
     return L
 
 
+def net_order(piv, steps: int, n: int) -> np.ndarray:
+    """Original index at each position after the swap history piv[0:steps] (0-based swaps)."""
+    order = np.arange(n)
+    for k in range(steps):
+        p = int(piv[k])
+        order[k], order[p] = order[p], order[k]
+    return order
+
+
 def batch_gesv_device(items, stream=None) -> Status:
     """One batched launch over a list of BatchItem (host structures holding device pointers)."""
     arr = (BatchItem * max(len(items), 1))(*items)
