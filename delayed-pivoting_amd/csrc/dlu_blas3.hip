@@ -59,48 +59,72 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(T *__restrict__ C, int
   const int wm = wave & 1, wn = wave >> 1;
 
   // Global loads are branch-free: an out-of-range element is read from a clamped in-range
-  // address and replaced by zero.  On the vector path M and K are multiples of the vector width,
-  // so a vector is either wholly inside or wholly outside.
+  // address, and replaced by zero only when it is stashed in LDS, after the current chunk's
+  // MFMAs, so the loads of the next chunk stay in flight while this one computes.  Row / column
+  // clamps are fixed per thread; the k clamp is needed only in a last, partial chunk.  On the
+  // vector path M and K are multiples of the vector width, so a vector is either wholly inside
+  // or wholly outside.
   typedef T dv_t __attribute__((ext_vector_type(VW)));
+  constexpr int AR = BM / VW, BR = BK / VW;  // threads per A column / per B column
+  static_assert(256 % AR == 0 && 256 % BR == 0, "load layout");
   dv_t ra[A_LD], rb[B_LD];
+  // A: every load of a thread reads the same row, k rows apart by 256 / AR.
+  const int a_row = row0 + (t % AR) * VW;
+  const bool a_ok = a_row < M;
+  const int a_k = t / AR;
+  const T *pa = A + static_cast<size_t>(a_ok ? a_row : 0);
+  // B: every load of a thread reads the same k, columns apart by 256 / BR.
+  const int b_k = (t % BR) * VW;
+  const int b_col = col0 + t / BR;
+  const T *pb = B + static_cast<size_t>(b_k);
   auto load = [&](int k0) {
+    const bool full = k0 + BK <= K;  // wave-uniform
 #pragma unroll
     for (int i = 0; i < A_LD; ++i) {
-      const int e = t + 256 * i;
-      const int row = row0 + (e % (BM / VW)) * VW, kk = k0 + e / (BM / VW);
-      const bool ok = row < M && kk < K;
-      const dv_t val = *reinterpret_cast<const dv_t *>(
-          A + static_cast<size_t>(ok ? row : 0) + static_cast<size_t>(ok ? kk : 0) * lda);
-      ra[i] = ok ? val : dv_t(T(0));
+      const int kk = k0 + a_k + i * (256 / AR);
+      const int kc = (full || kk < K) ? kk : 0;
+      ra[i] = *reinterpret_cast<const dv_t *>(pa + static_cast<size_t>(kc) * lda);
     }
 #pragma unroll
     for (int i = 0; i < B_LD; ++i) {
-      const int e = t + 256 * i;
-      const int kk = k0 + (e % (BK / VW)) * VW, col = col0 + e / (BK / VW);
-      const bool ok = col < N && kk < K;
-      const dv_t val = *reinterpret_cast<const dv_t *>(
-          B + static_cast<size_t>(ok ? kk : 0) + static_cast<size_t>(ok ? col : 0) * ldb);
-      rb[i] = ok ? val : dv_t(T(0));
+      const int col = b_col + i * (256 / BR);
+      const int kc = (full || k0 + b_k < K) ? k0 : -b_k;  // a k beyond K reads row 0
+      rb[i] = *reinterpret_cast<const dv_t *>(
+          pb + kc + static_cast<size_t>(col < N ? col : 0) * ldb);
     }
   };
-  auto stash = [&](int buf) {
+  auto stash = [&](int buf, int k0) {
 #pragma unroll
     for (int i = 0; i < A_LD; ++i) {
-      const int e = t + 256 * i;
-      *reinterpret_cast<dv_t *>(&As[buf][e / (BM / VW)][(e % (BM / VW)) * VW]) = ra[i];
+      const int kr = a_k + i * (256 / AR);
+      const bool ok = a_ok && k0 + kr < K;
+      *reinterpret_cast<dv_t *>(&As[buf][kr][(t % AR) * VW]) =
+          ok ? -ra[i] : dv_t(T(0));  // A is stored negated: acc = C + (-A) B
     }
 #pragma unroll
     for (int i = 0; i < B_LD; ++i) {
-      const int e = t + 256 * i;
-      *reinterpret_cast<dv_t *>(&Bs[buf][e / (BK / VW)][(e % (BK / VW)) * VW]) = rb[i];
+      const int cr = t / BR + i * (256 / BR);
+      const bool ok = b_col + i * (256 / BR) < N && k0 + b_k < K;
+      *reinterpret_cast<dv_t *>(&Bs[buf][cr][b_k]) = ok ? rb[i] : dv_t(T(0));
     }
   };
 
+  // The accumulators start from C (loaded together with the first chunk, so its latency is not
+  // paid again at the end) and the MFMAs add (-A) * B: the epilogue only stores.
   typename MF::acc_t acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = typename MF::acc_t{T(0), T(0), T(0), T(0)};
+    for (int j = 0; j < FN; ++j) {
+      const int col = col0 + wn * WN + 16 * j + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = row0 + wm * WM + 16 * i + MF::acc_row(lane, r);
+        const bool ok = row < M && col < N;
+        const T v = C[static_cast<size_t>(ok ? row : 0) + static_cast<size_t>(ok ? col : 0) * ldc];
+        acc[i][j][r] = ok ? v : T(0);
+      }
+    }
 
   T af[2][FM], bf[2][FN];
   auto frags = [&](int buf, int kk, int slot) {
@@ -114,7 +138,7 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(T *__restrict__ C, int
 
   const int nk = (K + BK - 1) / BK;
   load(0);
-  stash(0);
+  stash(0, 0);
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
@@ -131,7 +155,7 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(T *__restrict__ C, int
         for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[s & 1][i], bf[s & 1][j], acc[i][j]);
       __builtin_amdgcn_sched_barrier(0);
     }
-    if (kt + 1 < nk) stash(cur ^ 1);
+    if (kt + 1 < nk) stash(cur ^ 1, (kt + 1) * BK);
     __syncthreads();
   }
 
@@ -144,10 +168,7 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(T *__restrict__ C, int
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = row0 + wm * WM + 16 * i + MF::acc_row(lane, r);
-        if (row < M) {
-          T *cp = C + static_cast<size_t>(row) + static_cast<size_t>(col) * ldc;
-          *cp = *cp - acc[i][j][r];
-        }
+        if (row < M) C[static_cast<size_t>(row) + static_cast<size_t>(col) * ldc] = acc[i][j][r];
       }
     }
 }
