@@ -198,10 +198,11 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(T *__restrict__ a, int l
   __shared__ T Ls[NB][17];  // rows [r0+16, w) of L's current 16-column slice
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const int cnt = swap_list[4 * NB];
+  // No list: rows [pb, pb+w) in place (triangular solve only).
+  const int cnt = swap_list ? swap_list[4 * NB] : w;
   const int nx = cnt - w;  // displaced rows below the panel
   const int *dst = swap_list;
-  const int *src = swap_list + 2 * NB;
+  const int *src = swap_list ? swap_list + 2 * NB : nullptr;
   const int cs = c0 + blockIdx.x * NS;
   const int ncol = (c1 - cs) < NS ? (c1 - cs) : NS;
   const int nblk = (w + 15) / 16;
@@ -211,8 +212,9 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(T *__restrict__ a, int l
   // lane order for a fixed column).
   for (int idx = t; idx < cnt * NS; idx += 256) {
     const int i = idx % cnt, col = idx / cnt;
+    const int srow = src ? src[i] : pb + i;
     const T val =
-        col < ncol ? a[static_cast<size_t>(src[i]) + static_cast<size_t>(cs + col) * lda] : T(0);
+        col < ncol ? a[static_cast<size_t>(srow) + static_cast<size_t>(cs + col) * lda] : T(0);
     if (i < w)
       X[i][col] = val;
     else

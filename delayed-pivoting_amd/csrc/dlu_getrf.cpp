@@ -183,6 +183,7 @@ struct Run {
   int panels = 0;
   int max_groups = kMaxPanelGroups;  // workgroups the panel stream can hold at once
   int forced_rpl = 0;                // rows per lane requested by the caller (0: automatic)
+  int merge_rows = 1 << 30;          // pair panels while at least this many rows remain
   hipStream_t ps = nullptr, us = nullptr;  // internal panel / update streams
 
   // Opt-in launch timing: an event pair around each launch, on the launch's own stream.
@@ -275,21 +276,63 @@ struct Run {
 
   // Interchange + U block row + trailing update of columns [c0, c1) for the panel rows
   // [pb, pb+w), updating rows [pb+w, m).
-  hipError_t update(int pb, int wdt, int c0, int c1, const int *list, hipStream_t st) {
+  // A(r0:r0+M, c0:c0+N) -= A(r0:r0+M, kc:kc+K) * A(kr:kr+K, c0:c0+N).
+  hipError_t gemm(int r0, int c0, int kc, int kr, int M, int N, int K, hipStream_t st) {
+    if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+    const double flops = 2.0 * M * static_cast<double>(N) * K;
+    std::vector<Timed> &tv = gemm_uses_large_tiles(M, N) ? t_update_big : t_update;
+    hipError_t e = profile ? mark(tv, st, flops) : hipSuccess;
+    if (e == hipSuccess)
+      e = launch_gemm_sub<T>(at(a, lda, r0, c0), lda, at(a, lda, r0, kc), lda, at(a, lda, kr, c0),
+                             lda, M, N, K, st);
+    if (e == hipSuccess && profile) e = close(tv, st);
+    return e;
+  }
+
+  // Rows [pb, pb+w) of columns [c0, c1) become U12 = L11^{-1} (rows after the interchange of
+  // `list`; no list: in place).
+  hipError_t solve(int pb, int wdt, int c0, int c1, const int *list, hipStream_t st) {
     if (c1 <= c0 || wdt <= 0) return hipSuccess;
     hipError_t e = profile ? mark(t_solve, st, 0.0) : hipSuccess;
     if (e == hipSuccess) e = launch_swap_trsm(a, lda, pb, wdt, c0, c1, list, nb, st);
     if (e == hipSuccess && profile) e = close(t_solve, st);
-    if (e != hipSuccess) return e;
-    const int r0 = pb + wdt;
-    if (m - r0 <= 0) return hipSuccess;
-    const double flops = 2.0 * (m - r0) * static_cast<double>(c1 - c0) * wdt;
-    std::vector<Timed> &tv = gemm_uses_large_tiles(m - r0, c1 - c0) ? t_update_big : t_update;
-    if (profile && (e = mark(tv, st, flops)) != hipSuccess) return e;
-    e = launch_gemm_sub<T>(at(a, lda, r0, c0), lda, at(a, lda, r0, pb), lda, at(a, lda, pb, c0),
-                           lda, m - r0, c1 - c0, wdt, st);
-    if (e == hipSuccess && profile) e = close(tv, st);
     return e;
+  }
+
+  // Interchange + U block row + trailing update of columns [c0, c1) for the panel rows
+  // [pb, pb+w), updating rows [pb+w, m) (delayed_dgetrf.cpp:61-85, 101-125).
+  hipError_t update(int pb, int wdt, int c0, int c1, const int *list, hipStream_t st) {
+    if (c1 <= c0 || wdt <= 0) return hipSuccess;
+    hipError_t e = solve(pb, wdt, c0, c1, list, st);
+    if (e == hipSuccess) e = gemm(pb + wdt, c0, pb, pb, m - pb - wdt, c1 - c0, wdt, st);
+    return e;
+  }
+
+  // Two consecutive full panels share one trailing update.  After the first panel (pb0) the
+  // columns beyond the next panel get only its interchange and U block row; their GEMM waits.
+  // After the second panel (pb1 = pb0 + nb) those columns take the second panel's interchange,
+  // the first panel's update of the second panel's rows, the second panel's U block row, and
+  // then a single GEMM with K = 2 nb for all rows below (interchanges commute with the deferred
+  // update once the first panel's L is interchanged too).
+  struct Pending {
+    bool valid = false;
+    int pb = 0;  // first panel of the pair
+    int fe = 0;  // first column still owed the first panel's GEMM
+  };
+  hipError_t merged_update(const Pending &pd, int pb1, int c0, int c1, const int *list,
+                           hipStream_t st) {
+    if (c1 <= c0) return hipSuccess;
+    hipError_t e = launch_row_interchange(a, lda, c0, c1, list, nb, st);
+    if (e == hipSuccess) e = gemm(pb1, c0, pd.pb, pd.pb, nb, c1 - c0, nb, st);
+    if (e == hipSuccess) e = solve(pb1, nb, c0, c1, nullptr, st);
+    if (e == hipSuccess) e = gemm(pb1 + nb, c0, pd.pb, pd.pb, m - pb1 - nb, c1 - c0, 2 * nb, st);
+    return e;
+  }
+  // The deferred GEMM on its own, before the next panel's interchanges reach those columns.
+  hipError_t flush(Pending &pd, hipStream_t st) {
+    if (!pd.valid) return hipSuccess;
+    pd.valid = false;
+    return gemm(pd.pb + nb, pd.fe, pd.pb, pd.pb, m - pd.pb - nb, n - pd.fe, nb, st);
   }
 
   // Runs on the panel stream `ps` and the update stream `us`, forked from and joined back to the
@@ -303,11 +346,14 @@ struct Run {
     int pb = 0, parity = 0;
     int pe = std::min(pb + nb, steps);
     if ((e = panel(pb, pe, 0, parity)) != hipSuccess) return e;
+    Pending pend;
     for (;;) {
       if ((e = hipEventSynchronize(w.ev_ctl)) != hipSuccess) return e;
       const Control c = *w.h_ctl;
       if (c.error != 0) return hipErrorLaunchFailure;
       if (c.deferred) {
+        if ((e = hipStreamWaitEvent(us, w.ev_ctl, 0)) != hipSuccess) return e;
+        if ((e = flush(pend, us)) != hipSuccess) return e;
         *info = c.stop;
         break;
       }
@@ -315,9 +361,37 @@ struct Run {
       const int wdt = stop - pb;
       const int *list = w.d_lists + parity * kListStride;
       if ((e = hipStreamWaitEvent(us, w.ev_ctl, 0)) != hipSuccess) return e;
+      const bool full = !c.zero_column && wdt == nb && pe - pb == nb;
       if (!c.zero_column) {
         const int pe2 = std::min(pe + nb, steps);
-        if (pe < steps && lookahead) {
+        bool merged = false;
+        if (pend.valid && full && pe < steps && lookahead) {
+          // Second panel of a pair: the next panel's columns first, then the far columns.
+          if ((e = launch_row_interchange(a, lda, pend.pb, pb, list, nb, us)) != hipSuccess)
+            return e;
+          if ((e = merged_update(pend, pb, pe, pe2, list, us)) != hipSuccess) return e;
+          if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
+          if ((e = merged_update(pend, pb, pe2, n, list, us)) != hipSuccess) return e;
+          if ((e = launch_row_interchange(a, lda, 0, pend.pb, list, nb, us)) != hipSuccess)
+            return e;
+          pend.valid = false;
+          merged = true;
+        } else if ((e = flush(pend, us)) != hipSuccess) {
+          return e;
+        }
+        if (merged) {
+          // done: the next panel's columns are current and ev_next is recorded
+        } else if (pe < steps && lookahead && full && pe2 < n && m - pe2 >= merge_rows) {
+          // First panel of a pair: full update of the next panel's columns; the far columns get
+          // the interchange and U block row now and their GEMM after the next panel.
+          if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
+          if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
+          if ((e = solve(pb, wdt, pe2, n, list, us)) != hipSuccess) return e;
+          if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
+          pend.valid = true;
+          pend.pb = pb;
+          pend.fe = pe2;
+        } else if (pe < steps && lookahead) {
           // The next panel's columns first, then the far columns while the next panel runs.
           if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
           if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
@@ -337,6 +411,7 @@ struct Run {
       } else {
         // Zero column at `stop`: bring the far columns up to date with the accepted prefix
         // (delayed_dgetrf.cpp:183-186), search the whole active block, interchange, restart.
+        if ((e = flush(pend, us)) != hipSuccess) return e;
         if (wdt > 0) {
           if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
           if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
@@ -423,6 +498,7 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
     }
     run.forced_rpl = opt->rows_per_lane;
   }
+  if (opt && opt->merge_rows != 0) run.merge_rows = opt->merge_rows < 0 ? (1 << 30) : opt->merge_rows;
   {
     // CUs for the panel stream: enough for one resident workgroup per panel row block, at
     // least 16 and at most half the device (opt->panel_cus > 0 forces a size, < 0 disables).

@@ -30,7 +30,8 @@ class Status(enum.IntEnum):
 class Options(ctypes.Structure):
     _fields_ = [("nb", ctypes.c_int), ("lookahead", ctypes.c_int),
                 ("spin_limit_ticks", ctypes.c_longlong), ("profile", ctypes.c_int),
-                ("panel_cus", ctypes.c_int), ("rows_per_lane", ctypes.c_int)]
+                ("panel_cus", ctypes.c_int), ("rows_per_lane", ctypes.c_int),
+                ("merge_rows", ctypes.c_int)]
 
 
 class Report(ctypes.Structure):

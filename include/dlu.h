@@ -73,6 +73,8 @@ typedef struct dlu_options {
   int panel_cus;   /* CUs reserved for panel kernels while updates run on the rest:
                       0 = automatic, > 0 = that many, < 0 = no reservation */
   int rows_per_lane; /* panel rows held per lane: 0 = automatic, else 1, 2 or 4 */
+  int merge_rows;    /* pairs of panels share one trailing GEMM (K = 2 nb) while at least this
+                        many rows remain below them: 0 = automatic, < 0 = never */
 } dlu_options;
 
 /* What happened during one factorization. */
