@@ -174,126 +174,171 @@ __global__ __launch_bounds__(256, 2) void gemm_sub_kernel(T *__restrict__ C, int
 }
 
 // ---------------------------------------------------------------------------------------------
-// Row interchange + unit-lower triangular solve on a strip of 16 columns.
+// Row interchange + unit-lower triangular solve on a strip of 16 columns, one wave per strip.
 //
 // swap_list holds the panel's net row movement as (dst, src) pairs: entries [0, w) are rows
-// pb .. pb+w-1 in order, later entries are rows >= pb+w that received a row from the panel.
-// The strip's touched rows are gathered (all reads before any write), displaced rows are written
-// back in place, and rows [pb, pb+w) are solved in LDS:
+// pb .. pb+w-1 in order, later entries are rows >= pb+w that received a row from the panel (no
+// list: rows [pb, pb+w) in place, solve only).  The wave owns its 16 columns outright, so no
+// other wave touches them: it reads every touched row of the strip, writes the displaced rows
+// back, and solves
 //   X := L11^{-1} X  with L11 = A(pb:pb+w, pb:pb+w) unit lower,
-// by 16-row blocks on MFMA: X_b := inv(L_bb) X_b (the 16 x 16 diagonal inverses are formed by
-// every workgroup for itself), then X_rest -= L(rest, b) X_b with L's column slice staged in
-// LDS.  Rows at and beyond w are zero-padded and never written back.
+// with X held in registers as NB/16 MFMA result tiles: per 16-row block b, X_b := inv(L_bb) X_b,
+// then X_t -= L_tb X_b for every tile t below.  A result tile is the B operand of the next
+// product as it stands when k runs in the result's row order (k-step s of lane l is row
+// acc_row(l, s)); the A operand is read in that same k order.  The 16 x 16 diagonal inverses
+// are formed by the wave itself in LDS.  Rows at and beyond w are zero-padded and never
+// written back.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int NB>
-__global__ __launch_bounds__(256) void swap_trsm_kernel(T *__restrict__ a, int lda, int pb,
-                                                       int w, int c0, int c1,
-                                                       const int *__restrict__ swap_list) {
+template <typename T, int NB, bool LIST, bool FULL>
+__global__ __launch_bounds__(64) void swap_trsm_kernel(T *__restrict__ a, int lda, int pb, int w,
+                                                      int c0, int c1,
+                                                      const int *__restrict__ swap_list) {
   typedef Mfma<T> MF;
-  constexpr int NS = 16;
-  constexpr int XS = NS + 2;  // row stride of X
-  __shared__ T X[NB][XS];
-  __shared__ T E[NB][NS + 1];
-  __shared__ T invL[NB / 16][16][17];
-  __shared__ T Ls[NB][17];  // rows [r0+16, w) of L's current 16-column slice
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  // No list: rows [pb, pb+w) in place (triangular solve only).
-  const int cnt = swap_list ? swap_list[4 * NB] : w;
-  const int nx = cnt - w;  // displaced rows below the panel
-  const int *dst = swap_list;
-  const int *src = swap_list ? swap_list + 2 * NB : nullptr;
-  const int cs = c0 + blockIdx.x * NS;
-  const int ncol = (c1 - cs) < NS ? (c1 - cs) : NS;
-  const int nblk = (w + 15) / 16;
+  typedef typename MF::acc_t acc_t;
+  constexpr int NT = NB / 16;  // 16-row tiles of the panel
+  constexpr int NJ = NB / 64;  // rows per lane in a 64-row sweep
+  __shared__ T X[NB][17];
+  __shared__ T D[NT][16][17];  // diagonal blocks of L11, then their inverses
+  const int l = threadIdx.x;
+  const int lr = l & 15;
+  const int cnt = LIST ? swap_list[4 * NB] : w;
+  const int *src = swap_list + 2 * NB;  // read only with a list
+  const int cs = c0 + blockIdx.x * 16;
+  const int ncol = (c1 - cs) < 16 ? (c1 - cs) : 16;
   const T *L = a + static_cast<size_t>(pb) + static_cast<size_t>(pb) * lda;
+  T *S = a + static_cast<size_t>(cs) * lda;  // column cs: the strip
 
-  // Gather every touched row of the strip before anything is written (rows are contiguous in
-  // lane order for a fixed column).
-  for (int idx = t; idx < cnt * NS; idx += 256) {
-    const int i = idx % cnt, col = idx / cnt;
-    const int srow = src ? src[i] : pb + i;
-    const T val =
-        col < ncol ? a[static_cast<size_t>(srow) + static_cast<size_t>(cs + col) * lda] : T(0);
-    if (i < w)
-      X[i][col] = val;
-    else
-      E[i - w][col] = val;
+  // 1. Every read of the strip before any write: rows [0, w) after the interchange into LDS,
+  // displaced rows into registers.  Lanes run down a column, 64 consecutive rows per load.
+  // Columns past the strip's end read column cmax instead and are never stored (no column of X
+  // feeds another).  FULL (w == NB) needs no row mask, so no loaded value goes through a select
+  // and the loads issue back to back; a partial panel zeroes rows and columns at and beyond w.
+  const int cmax = ncol - 1;
+  int xsrc[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int i = 64 * j + l;
+    const int from = LIST ? src[i] : pb + i;  // i < NB: inside the list
+    xsrc[j] = i < w ? from : -1;
   }
-  for (int idx = t; idx < (nblk * 16 - w) * NS; idx += 256) X[w + idx / NS][idx % NS] = T(0);
-
-  // Inverses of the diagonal blocks: thread (blk, col) solves L_bb x = e_col.
-  if (t < 16 * nblk) {
-    const int blk = t >> 4, col = t & 15;
-    const int r0 = blk * 16;
-    const int nr = (w - r0) < 16 ? (w - r0) : 16;
-    T x[16];
+  T e[NJ][16];
+  int edst[NJ];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      T sum = (r == col) ? T(1) : T(0);
-      if (r > col && r < nr) {
+  for (int j = 0; j < NJ; ++j) {
+    const int idx = w + 64 * j + l;  // < 2 NB: inside the list; displaced rows number <= w
+    const bool ok = LIST && idx < cnt;
+    const int to = LIST ? swap_list[idx] : -1, from = LIST ? src[idx] : pb;
+    edst[j] = ok ? to : -1;
+    const int es = ok ? from : pb;
 #pragma unroll
-        for (int q = 0; q < 16; ++q)
-          if (q >= col && q < r)
-            sum -= L[static_cast<size_t>(r0 + r) + static_cast<size_t>(r0 + q) * lda] * x[q];
-      }
-      x[r] = (r < nr) ? sum : T(0);
+    for (int c = 0; c < 16; ++c)
+      e[j][c] = S[static_cast<size_t>(es) + static_cast<size_t>(c < cmax ? c : cmax) * lda];
+  }
+#pragma unroll
+  for (int c = 0; c < 16; ++c)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int row = (FULL || xsrc[j] >= 0) ? xsrc[j] : pb;
+      const T v = S[static_cast<size_t>(row) + static_cast<size_t>(c < cmax ? c : cmax) * lda];
+      X[64 * j + l][c] = (FULL || xsrc[j] >= 0) ? v : T(0);
     }
+  // Diagonal blocks of L11 (zero outside [0, w)), 16 consecutive rows per column segment.
 #pragma unroll
-    for (int r = 0; r < 16; ++r) invL[blk][r][col] = x[r];
+  for (int j = 0; j < 4 * NT; ++j) {
+    const int idx = 64 * j + l;
+    const int b = idx >> 8, q = (idx >> 4) & 15, r = idx & 15;
+    const int row = 16 * b + r, col = 16 * b + q;
+    const bool ok = FULL || (row < w && col < w);
+    const T v = L[ok ? static_cast<size_t>(row) + static_cast<size_t>(col) * lda : 0];
+    D[b][r][q] = ok ? v : T(0);
+  }
+  __syncthreads();  // the strip's reads have all returned (their values sit in LDS)
+
+  // 2. Displaced rows go back in place.
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+    if (edst[j] >= 0)
+#pragma unroll
+      for (int c = 0; c < 16; ++c)
+        if (c < ncol) S[static_cast<size_t>(edst[j]) + static_cast<size_t>(c) * lda] = e[j][c];
+
+  // 3. inv(L_bb), column lr of block b per lane: forward substitution on e_lr.  Blocks past w
+  // are zero, so their inverse is zero and the solve below needs no branch on w.
+  T x[NT / 4][16];
+#pragma unroll
+  for (int h = 0; h < NT / 4; ++h) {
+    const int b = (l >> 4) + 4 * h;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x[h][r] = (r == lr) ? T(1) : T(0);
+#pragma unroll
+    for (int q = 0; q < 15; ++q)
+#pragma unroll
+      for (int r = q + 1; r < 16; ++r) x[h][r] -= D[b][r][q] * x[h][q];
+    if (!FULL) {
+      const int nr = w - 16 * b;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) x[h][r] = (r < nr) ? x[h][r] : T(0);
+    }
   }
   __syncthreads();
-  for (int idx = t; idx < nx * ncol; idx += 256) {
-    const int i = idx % nx, col = idx / nx;
-    a[static_cast<size_t>(dst[w + i]) + static_cast<size_t>(cs + col) * lda] = E[i][col];
+#pragma unroll
+  for (int h = 0; h < NT / 4; ++h)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) D[(l >> 4) + 4 * h][r][lr] = x[h][r];
+  __syncthreads();
+
+  // 4. The solve on MFMA with X in registers.  The A operands of block b + 1 (its column slice
+  // of L, negated) are loaded while block b's products run.
+  acc_t xr[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) xr[t][r] = X[16 * t + MF::acc_row(l, r)][lr];
+  auto lslice = [&](int b, int t, int s) -> T {
+    const int row = 16 * t + lr, k = 16 * b + MF::acc_row(l, s);
+    const bool ok = FULL || (row < w && k < w);
+    const T v = L[ok ? static_cast<size_t>(row) + static_cast<size_t>(k) * lda : 0];
+    return ok ? -v : T(0);
+  };
+  T lcur[NT][4];
+#pragma unroll
+  for (int t = 1; t < NT; ++t)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) lcur[t][s] = lslice(0, t, s);
+#pragma unroll
+  for (int b = 0; b < NT; ++b) {
+    T lnext[NT][4];
+#pragma unroll
+    for (int t = b + 2; t < NT; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) lnext[t][s] = lslice(b + 1, t, s);
+    acc_t d = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) d = MF::mma(D[b][lr][MF::acc_row(l, s)], xr[b][s], d);
+    xr[b] = d;
+#pragma unroll
+    for (int t = b + 1; t < NT; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) xr[t] = MF::mma(lcur[t][s], xr[b][s], xr[t]);
+#pragma unroll
+    for (int t = b + 2; t < NT; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) lcur[t][s] = lnext[t][s];
   }
 
-  const int lr = lane & 15, lk = lane >> 4;
-  for (int blk = 0; blk < nblk; ++blk) {
-    const int r0 = blk * 16;
-    const int below = nblk * 16 - (r0 + 16);  // padded rows under the block
-    // Stage L(r0+16 .. w, r0 .. r0+16) (zero beyond w) while wave 0 solves the diagonal block.
-    for (int idx = t; idx < below * 16; idx += 256) {
-      const int rr = idx % below, q = idx / below;
-      const int row = r0 + 16 + rr, colq = r0 + q;
-      Ls[rr][q] = (row < w && colq < w)
-                      ? L[static_cast<size_t>(row) + static_cast<size_t>(colq) * lda]
-                      : T(0);
+  // 5. Rows [pb, pb+w) of the strip, through LDS so the stores run down columns.
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) X[16 * t + MF::acc_row(l, r)][lr] = xr[t][r];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 16; ++c)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int i = 64 * j + l;
+      if (i < w && c < ncol) S[static_cast<size_t>(pb + i) + static_cast<size_t>(c) * lda] = X[i][c];
     }
-    if (wave == 0) {
-      typename MF::acc_t d = {T(0), T(0), T(0), T(0)};
-#pragma unroll
-      for (int k4 = 0; k4 < 16; k4 += 4) {
-        const T av = invL[blk][lr][k4 + lk];
-        const T bv = X[r0 + k4 + lk][lr];
-        d = MF::mma(av, bv, d);
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) X[r0 + MF::acc_row(lane, r)][lr] = d[r];
-    }
-    __syncthreads();
-    // X(rows below) -= L(rows below, block) * X_b, one 16 x 16 tile per wave at a time.
-    for (int tb = wave; tb < below / 16; tb += 4) {
-      const int rt = r0 + 16 + 16 * tb;
-      typename MF::acc_t d;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) d[r] = X[rt + MF::acc_row(lane, r)][lr];
-#pragma unroll
-      for (int k4 = 0; k4 < 16; k4 += 4) {
-        const T av = -Ls[16 * tb + lr][k4 + lk];
-        const T bv = X[r0 + k4 + lk][lr];
-        d = MF::mma(av, bv, d);
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) X[rt + MF::acc_row(lane, r)][lr] = d[r];
-    }
-    __syncthreads();
-  }
-
-  for (int idx = t; idx < w * ncol; idx += 256) {
-    const int i = idx % w, col = idx / w;
-    a[static_cast<size_t>(pb + i) + static_cast<size_t>(cs + col) * lda] = X[i][col];
-  }
 }
 
 template <typename T, int BM, int BN, int BK>
@@ -329,13 +374,29 @@ template <typename T>
 hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const int *swap_list,
                             int nb, hipStream_t s) {
   if (c1 <= c0 || w <= 0) return hipSuccess;
-  const dim3 grid((c1 - c0 + 15) / 16), block(256);
+  const dim3 grid((c1 - c0 + 15) / 16), block(64);
+#define DLU_TRSM_CASE(NBV)                                                                      \
+  case NBV:                                                                                      \
+    if (swap_list && w == NBV)                                                                   \
+      hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, true, true>), grid, block, 0, s, a, lda, pb,  \
+                         w, c0, c1, swap_list);                                                  \
+    else if (swap_list)                                                                          \
+      hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, true, false>), grid, block, 0, s, a, lda, pb, \
+                         w, c0, c1, swap_list);                                                  \
+    else if (w == NBV)                                                                           \
+      hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, false, true>), grid, block, 0, s, a, lda, pb, \
+                         w, c0, c1, swap_list);                                                  \
+    else                                                                                         \
+      hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, false, false>), grid, block, 0, s, a, lda,    \
+                         pb, w, c0, c1, swap_list);                                              \
+    break;
   switch (nb) {
-    case 64: hipLaunchKernelGGL((swap_trsm_kernel<T, 64>), grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
-    case 128: hipLaunchKernelGGL((swap_trsm_kernel<T, 128>), grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
-    case 256: hipLaunchKernelGGL((swap_trsm_kernel<T, 256>), grid, block, 0, s, a, lda, pb, w, c0, c1, swap_list); break;
+    DLU_TRSM_CASE(64)
+    DLU_TRSM_CASE(128)
+    DLU_TRSM_CASE(256)
     default: return hipErrorInvalidValue;
   }
+#undef DLU_TRSM_CASE
   return hipGetLastError();
 }
 

@@ -73,6 +73,7 @@ struct Workspace {
   Streams streams[4];
   int nstreams = 0;
   hipEvent_t ev_fork = nullptr, ev_ctl = nullptr, ev_next = nullptr, ev_join = nullptr;
+  hipEvent_t ev_solved = nullptr, ev_tail = nullptr;  // far U block row done / tail GEMM done
   bool ready = false;
 };
 
@@ -106,7 +107,7 @@ hipError_t prepare(Workspace &w, int device, int steps) {
     if ((e = hipMalloc(&w.d_pv, kSearchPartials * sizeof(double))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_po, kSearchPartials * sizeof(unsigned long long))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_phase, 8 * sizeof(unsigned long long))) != hipSuccess) return e;
-    for (hipEvent_t *ev : {&w.ev_fork, &w.ev_ctl, &w.ev_next, &w.ev_join})
+    for (hipEvent_t *ev : {&w.ev_fork, &w.ev_ctl, &w.ev_next, &w.ev_join, &w.ev_solved, &w.ev_tail})
       if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
     w.ready = true;
@@ -184,6 +185,8 @@ struct Run {
   int max_groups = kMaxPanelGroups;  // workgroups the panel stream can hold at once
   int forced_rpl = 0;                // rows per lane requested by the caller (0: automatic)
   int merge_rows = 1 << 30;          // pair panels while at least this many rows remain
+  bool split = true;                 // tail of the far GEMM on the panel stream's CUs
+  int panel_cus = 0;                 // CUs of the panel stream (0: no partition)
   hipStream_t ps = nullptr, us = nullptr;  // internal panel / update streams
 
   // Opt-in launch timing: an event pair around each launch, on the launch's own stream.
@@ -335,6 +338,50 @@ struct Run {
     return gemm(pd.pb + nb, pd.fe, pd.pb, pd.pb, m - pd.pb - nb, n - pd.fe, nb, st);
   }
 
+  // Far-update tail on the panel stream.  While the trailing GEMM outlasts the next panel, the
+  // panel stream's CUs would idle once that panel is done; the GEMM's last columns [cb, n) are
+  // queued there behind it instead.  The split balances the two streams with a throughput model
+  // (per-CU MFMA rate of the large-tile GEMM, per-column panel step latency, U block row solve
+  // per column):
+  //   update stream: solve + GEMM(pe2 .. cb) on U CUs;  panel stream: panel + GEMM(cb .. n) on P.
+  struct Tail {
+    bool queued = false;   // a tail GEMM is waiting to be launched behind the next panel
+    bool pending = false;  // a launched tail GEMM may still run (ev_tail marks its end)
+    int pb = 0, wdt = 0, cb = 0;
+  };
+  Tail tail;
+  int tail_split(int pb, int wdt, int c0, int next_wdt) const {
+    if (!split || panel_cus <= 0 || c0 >= n) return n;
+    const int rows = m - pb - wdt;
+    const int U = std::max(1, w.cus - panel_cus), P = panel_cus;
+    const double rate = sizeof(T) == 8 ? 0.135e12 : 0.27e12;     // flop/s per CU
+    const double work = 2.0 * rows * static_cast<double>(n - c0) * wdt;
+    const double t_panel = 5.2e-6 * next_wdt, t_solve = 7.5e-9 * (n - c0);
+    const double t = (work / rate + P * t_panel + U * t_solve) / (P + U);
+    const double w_tail = P * rate * (t - t_panel);
+    if (w_tail <= 0.0 || rows <= 0) return n;
+    const int cols = static_cast<int>(w_tail / (2.0 * rows * wdt)) / 128 * 128;
+    if (cols < 128 || n - cols < c0 + nb) return n;  // the next panel's block stays on `us`
+    return n - cols;
+  }
+  hipError_t wait_tail(hipStream_t st) {
+    if (!tail.pending) return hipSuccess;
+    tail.pending = false;
+    return hipStreamWaitEvent(st, w.ev_tail, 0);
+  }
+  // Called right after the next panel is queued on `ps`.
+  hipError_t launch_tail() {
+    if (!tail.queued) return hipSuccess;
+    tail.queued = false;
+    hipError_t e = hipStreamWaitEvent(ps, w.ev_solved, 0);
+    if (e == hipSuccess)
+      e = gemm(tail.pb + tail.wdt, tail.cb, tail.pb, tail.pb, m - tail.pb - tail.wdt, n - tail.cb,
+               tail.wdt, ps);
+    if (e == hipSuccess) e = hipEventRecord(w.ev_tail, ps);
+    if (e == hipSuccess) tail.pending = true;
+    return e;
+  }
+
   // Runs on the panel stream `ps` and the update stream `us`, forked from and joined back to the
   // caller's stream.  Per panel the host reads the control block once to learn where the panel
   // stopped; every pivot decision inside the panel is taken on the device.
@@ -353,6 +400,7 @@ struct Run {
       if (c.error != 0) return hipErrorLaunchFailure;
       if (c.deferred) {
         if ((e = hipStreamWaitEvent(us, w.ev_ctl, 0)) != hipSuccess) return e;
+        if ((e = wait_tail(us)) != hipSuccess) return e;
         if ((e = flush(pend, us)) != hipSuccess) return e;
         *info = c.stop;
         break;
@@ -365,6 +413,10 @@ struct Run {
       if (!c.zero_column) {
         const int pe2 = std::min(pe + nb, steps);
         bool merged = false;
+        const bool plain = !pend.valid && pe < steps && lookahead &&
+                           !(full && pe2 < n && m - pe2 >= merge_rows);
+        // Everything but the plain look-ahead step touches far columns before anything else.
+        if (!plain && (e = wait_tail(us)) != hipSuccess) return e;
         if (pend.valid && full && pe < steps && lookahead) {
           // Second panel of a pair: the next panel's columns first, then the far columns.
           if ((e = launch_row_interchange(a, lda, pend.pb, pb, list, nb, us)) != hipSuccess)
@@ -392,10 +444,22 @@ struct Run {
           pend.pb = pb;
           pend.fe = pe2;
         } else if (pe < steps && lookahead) {
-          // The next panel's columns first, then the far columns while the next panel runs.
+          // The next panel's columns first, then the far columns while the next panel runs
+          // (their last columns on the panel stream behind it when the GEMM outlasts it).
           if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
           if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
-          if ((e = update(pb, wdt, pe2, n, list, us)) != hipSuccess) return e;
+          if ((e = wait_tail(us)) != hipSuccess) return e;
+          if ((e = solve(pb, wdt, pe2, n, list, us)) != hipSuccess) return e;
+          const int cb = tail_split(pb, wdt, pe2, pe2 - pe);
+          if (cb < n) {
+            if ((e = hipEventRecord(w.ev_solved, us)) != hipSuccess) return e;
+            tail.queued = true;
+            tail.pb = pb;
+            tail.wdt = wdt;
+            tail.cb = cb;
+          }
+          if ((e = gemm(pb + wdt, pe2, pb, pb, m - pb - wdt, cb - pe2, wdt, us)) != hipSuccess)
+            return e;
           if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
         } else {
           if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
@@ -411,6 +475,7 @@ struct Run {
       } else {
         // Zero column at `stop`: bring the far columns up to date with the accepted prefix
         // (delayed_dgetrf.cpp:183-186), search the whole active block, interchange, restart.
+        if ((e = wait_tail(us)) != hipSuccess) return e;
         if ((e = flush(pend, us)) != hipSuccess) return e;
         if (wdt > 0) {
           if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
@@ -429,6 +494,7 @@ struct Run {
       parity ^= 1;
       if ((e = hipStreamWaitEvent(ps, w.ev_next, 0)) != hipSuccess) return e;
       if ((e = panel(pb, pe, c.zero_column ? 1 : 0, parity)) != hipSuccess) return e;
+      if ((e = launch_tail()) != hipSuccess) return e;
     }
     return hipSuccess;
   }
@@ -499,6 +565,7 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
     run.forced_rpl = opt->rows_per_lane;
   }
   if (opt && opt->merge_rows != 0) run.merge_rows = opt->merge_rows < 0 ? (1 << 30) : opt->merge_rows;
+  run.split = !(opt && opt->split_update < 0);
   {
     // CUs for the panel stream: enough for one resident workgroup per panel row block, at
     // least 16 and at most half the device (opt->panel_cus > 0 forces a size, < 0 disables).
@@ -517,6 +584,7 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
     }
     run.ps = st->panel;
     run.us = st->update;
+    run.panel_cus = std::max(0, st->panel_cus);
     if (st->panel_cus > 0) run.max_groups = std::min(kMaxPanelGroups, st->panel_cus);
   }
   if (run.profile && (e = hipMemsetAsync(w.d_phase, 0, 8 * sizeof(unsigned long long), stream)) !=

@@ -31,7 +31,7 @@ class Options(ctypes.Structure):
     _fields_ = [("nb", ctypes.c_int), ("lookahead", ctypes.c_int),
                 ("spin_limit_ticks", ctypes.c_longlong), ("profile", ctypes.c_int),
                 ("panel_cus", ctypes.c_int), ("rows_per_lane", ctypes.c_int),
-                ("merge_rows", ctypes.c_int)]
+                ("merge_rows", ctypes.c_int), ("split_update", ctypes.c_int)]
 
 
 class Report(ctypes.Structure):
