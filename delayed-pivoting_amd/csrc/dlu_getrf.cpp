@@ -85,7 +85,7 @@ constexpr int kListStride = 4 * 256 + 4;  // swap_list_ints(256), enough for eve
 constexpr size_t slot_bytes() {
   return 2ull * kMaxPanelGroups * static_cast<size_t>(slot_words(256)) * sizeof(uint64_t);
 }
-constexpr size_t rowk_bytes() { return 2ull * 2 * 256 * sizeof(uint64_t); }
+constexpr size_t rowk_bytes() { return 2ull * rowk_words(256) * sizeof(uint64_t); }
 
 std::mutex g_lock;
 Workspace g_ws[kMaxDevices];
@@ -602,8 +602,11 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   Control init{};
   init.min_margin = 2.0;
   *w.h_ctl = init;
+  // Swap lists start empty with their append counters at zero (panels clear them after use;
+  // this also recovers from an aborted call).
   if ((e = hipMemcpyAsync(w.d_ctl, w.h_ctl, sizeof(Control), hipMemcpyHostToDevice, stream)) ==
           hipSuccess &&
+      (e = hipMemsetAsync(w.d_lists, 0, 2 * kListStride * sizeof(int), stream)) == hipSuccess &&
       (e = launch_init_steps(w.d_ipiv, w.d_jpiv, w.d_margins, steps, stream)) == hipSuccess) {
     int out_info = steps;
     e = run.factor(&out_info, lookahead);

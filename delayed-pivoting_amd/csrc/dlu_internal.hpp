@@ -66,10 +66,14 @@ struct Mfma<float> {
   static __device__ __forceinline__ int acc_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-// A panel's net row movement: [0, 2nb) destination rows, [2nb, 4nb) source rows, [4nb] count.
-// Two such lists alternate between consecutive panels so the far update of panel j can still
-// read its list while panel j+1 writes the next one.
+// A panel's net row movement: [0, 2nb) destination rows, [2nb, 4nb) source rows, [4nb] count;
+// [4nb+1] and [4nb+2] are the panel kernel's append counter and finished-workgroup count (zero
+// between panels).  Two such lists alternate between consecutive panels so the far update of
+// panel j can still read its list while panel j+1 writes the next one.
 inline constexpr int swap_list_ints(int nb) { return 4 * nb + 4; }
+// Row k exchange area per step parity: row k's values (at most 2 words per element), then the
+// original index of the row whose content it holds.
+inline constexpr int rowk_words(int nb) { return 2 * nb + 8; }
 
 template <typename T>
 struct PanelParams {
@@ -85,7 +89,7 @@ struct PanelParams {
   double *margins;     // device per-step relative top-2 gap (may be null)
   Control *ctl;
   uint64_t *slots;     // [2][groups][slot_words(nb)]
-  uint64_t *rowk;      // [2][2*nb]
+  uint64_t *rowk;      // [2][rowk_words(nb)]
   int *swap_list;      // swap_list_ints(nb) ints for this panel (parity selected by the host)
   long long spin_limit;  // bounded wait, in s_memrealtime ticks (100 MHz)
   unsigned long long *phase_ticks;  // optional [8] per-phase tick totals (workgroup 0)

@@ -21,11 +21,17 @@
 // NB values; the owner of row k also publishes row k.  Every workgroup then reduces all candidates
 // (largest magnitude, lowest row on ties -- BLAS idamax semantics), reads the winner's row and
 // row k, performs the interchange on its own rows, scales its part of column k by the reciprocal
-// of the pivot and applies the rank-1 update to its part of the panel.  Exchange words carry a
-// 32-bit payload and the 32-bit sequence number of the step that wrote them (an LL-style
-// protocol), so readers spin on relaxed device-scope loads and no cache writeback/invalidate
-// fence sits on the per-step path.  Every wait is bounded (PanelParams::spin_limit); a
-// workgroup whose wait expires raises ctl->abort_flag and all workgroups leave.
+// of the pivot and applies the rank-1 update to its part of the panel.  Every exchange word
+// carries the 16-bit tag of the step that wrote it next to its payload (an LL-style protocol:
+// header words hold 48 payload bits, data words 32), so readers spin on relaxed device-scope
+// loads and no cache writeback/invalidate fence sits on the per-step path.  Every wait is
+// bounded (PanelParams::spin_limit); a workgroup whose wait expires raises ctl->abort_flag and
+// all workgroups leave.
+//
+// Net row movement.  Each workgroup also keeps, per row it holds, the original index of the row
+// whose content sits there (in LDS, read and written only by its last wave); the candidate's and
+// row k's origins travel in the same exchange, so at the end every workgroup lists its own moved
+// rows in parallel (an atomic append) instead of one lane replaying the pivot history.
 #include "dlu_internal.hpp"
 
 #include <climits>
@@ -244,6 +250,12 @@ __device__ __forceinline__ uint64_t wave_max_key(uint64_t key) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// An integer carried bit for bit through the exchange as a T.
+__device__ __forceinline__ void as_word(double &x, int i) { x = mk(static_cast<unsigned>(i), 0u); }
+__device__ __forceinline__ void as_word(float &x, int i) { x = __int_as_float(i); }
+__device__ __forceinline__ int from_word(double x) { return static_cast<int>(lo32(x)); }
+__device__ __forceinline__ int from_word(float x) { return __float_as_int(x); }
+
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -271,22 +283,23 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
   // s_win[2..3]: columns 0 and 1 at the start.
   __shared__ T s_win[4][R];
   __shared__ T s_u[NW][16];        // per-wave slice of the pivot row
+  __shared__ T s_lp[NW][R];        // per-wave copy of the pending step's multipliers
   __shared__ T s_k[NW][16];        // per-wave slice of the old row k (owner of row p)
   __shared__ T s_pub[NW][2][16];   // per-wave slices of the candidate row and of row k
   __shared__ uint64_t s_ck[2][kMaxPanelGroups];  // candidate keys, alternating by step
   __shared__ int s_crow[2][kMaxPanelGroups];
   __shared__ int s_fail;
   __shared__ int s_hist[NB];
+  __shared__ int s_orig[NB];    // per step: original index of the row that became row k
+  __shared__ int s_from[R];     // per held row: original index of the content it carries
   __shared__ double s_g1[NB];   // per step: largest and second-largest |value| (workgroup 0)
   __shared__ double s_g2[NB];
-  __shared__ int s_cur[NB];
-  __shared__ int s_out_row[NB];
-  __shared__ int s_out_cur[NB];
 
   row16_t v[RPL];
 #pragma unroll
   for (int r = 0; r < RPL; ++r) {
     const int row = base + 64 * r + lane;
+    if (wave == NW - 1) s_from[64 * r + lane] = row;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int col = col0 + j;
@@ -322,15 +335,46 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
   int zero_col = 0;
   double min_margin = 2.0;
   int near_ties = 0, exact_ties = 0;
+  // The rank-1 update of a step reaches the bulk of the panel (all but the window) in the next
+  // step, after that step's header is out: the pending multipliers s_lp[wave] and the pivot row
+  // slice s_u[wave] (both private to the wave) are applied while the header travels, before the
+  // candidate row is staged.
+  int jp = 0;         // this wave's column index of the pending step's column (wave-uniform)
+  bool pend = false;  // a step's bulk update is pending
+#define DLU_APPLY_PENDING()                                                          \
+  if (pend) {                                                                        \
+    pend = false;                                                                    \
+    T u[16];                                                                         \
+    _Pragma("unroll") for (int j = 0; j < 16; ++j) u[j] = s_u[wave][j];              \
+    _Pragma("unroll") for (int r = 0; r < RPL; ++r) {                                \
+      const T l = s_lp[wave][64 * r + lane];                                         \
+      if (jp < 0) {                                                                  \
+        _Pragma("unroll") for (int j = 0; j < 16; ++j) v[r][j] = fma(-l, u[j], v[r][j]); \
+      } else if (jp < 16) {                                                          \
+        _Pragma("unroll") for (int j = 0; j < 16; ++j) {                             \
+          if (j == jp) v[r][j] = l;                                                  \
+          if (j > jp) v[r][j] = fma(-l, u[j], v[r][j]);                              \
+        }                                                                            \
+      }                                                                              \
+    }                                                                                \
+  }
+  // Margin bookkeeping of the previous step, finished by wave 1 of workgroup 0 while wave 0
+  // polls the next step's headers (off the per-step path).
+  int prev_gwin = -1;
+  double prev_w2 = 0.0;
   // Optional phase timing (workgroup 0, thread 0; 100 MHz ticks).
   const bool timed = P.phase_ticks != nullptr && g == 0 && threadIdx.x == 0;
-  unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  unsigned long long tp = timed ? wall_clock64() : 0;
+  // Totals live in LDS (s_ph[0..7], s_ph[8] = last stamp) so they cost no registers.
+  __shared__ unsigned long long s_ph[9];
+  if (timed) {
+    for (int i = 0; i < 8; ++i) s_ph[i] = 0;
+    s_ph[8] = wall_clock64();
+  }
 #define DLU_PHASE(i)                              \
   if (timed) {                                    \
     const unsigned long long tn = wall_clock64(); \
-    ph[i] += tn - tp;                             \
-    tp = tn;                                      \
+    s_ph[i] += tn - s_ph[8];                      \
+    s_ph[8] = tn;                                 \
   }
 
   for (int c = 0; c < width; ++c) {
@@ -361,6 +405,13 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
     const int lrow = wave_min(myrow);
     DLU_PHASE(0)
 
+    // The header first: it is what every workgroup polls on.  The candidate row's data words
+    // are read only after the reduction, so they follow once the pending update is applied.
+    uint64_t *slot = P.slots + (static_cast<size_t>(par) * P.groups + g) * SW;
+    uint64_t *rk = P.rowk + static_cast<size_t>(par) * rowk_words(NB);
+    if (wave == NW - 1 && lane == 0) hdr_put(slot, seq, key_value(lk), lrow);
+    DLU_APPLY_PENDING()
+
     // Stage this wave's 16 columns of the candidate row and of row k in LDS (so 32 lanes each
     // store one exchange word), and hand column c+2 (current through step c-1) to every wave.
 #pragma unroll
@@ -387,10 +438,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
     wave_sync_lds();
 
     // Publish: lanes 0-31 store this wave's slice of the candidate row, lanes 32-63 that of
-    // row k, one exchange word each; the last wave stores the header (the polling waves do not
-    // wait on it).
-    uint64_t *slot = P.slots + (static_cast<size_t>(par) * P.groups + g) * SW;
-    uint64_t *rk = P.rowk + static_cast<size_t>(par) * 2 * NB;
+    // row k, one exchange word each.
     {
       const int j = (lane & 31) / WPV, part = lane % WPV;
       const bool cand_here = lrow >= base && lrow < base + R;
@@ -401,7 +449,16 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
       }
     }
     if (wave == NW - 1) {
-      if (lane == 0) hdr_put(slot, seq, key_value(lk), lrow);
+      // The candidate's origin in the slot's spare words, row k's next to its values.
+      T ow;
+      if (lane == 1 && lrow >= base && lrow < base + R) {
+        as_word(ow, s_from[lrow - base]);
+        ll_put_value(slot + 4, seq, ow);
+      }
+      if (lane == 2 && k_here) {
+        as_word(ow, s_from[k - base]);
+        ll_put_value(rk + 2 * NB, seq, ow);
+      }
       // The second-largest |value| of this workgroup's rows, for the reported margins only:
       // published after the header, off the path every workgroup waits on.
       uint64_t m2 = 0;
@@ -411,6 +468,20 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
       if (lane == 0) ll_put_value(slot + 2, seq, static_cast<T>(key_value(m2)));
     }
     DLU_PHASE(1)
+
+    // Margin of the previous step: the largest candidate outside the winner's workgroup.
+    if (g == 0 && wave == 1 && prev_gwin >= 0) {
+      constexpr int GQ = kMaxPanelGroups / 64;
+      uint64_t o = 0;
+#pragma unroll
+      for (int q = 0; q < GQ; ++q) {
+        const int gi = lane + 64 * q;
+        const uint64_t kq = s_ck[cb ^ 1][gi];
+        o = max(o, (gi < P.groups && gi != prev_gwin) ? kq : 0ull);
+      }
+      const double gs = key_value(wave_max_key(o));
+      if (lane == 0) s_g2[c - 1] = fmax(gs, prev_w2);
+    }
 
     // Gather every workgroup's header.
     if (threadIdx.x < P.groups) {
@@ -456,20 +527,14 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
       for (int q = 0; q < GQ; ++q) gr = (gk[q] == g1k) ? min(gr, gq[q]) : gr;
       const int p = wave_min(gr);
       const int gwin = (p - P.pb) / R;  // rows are dealt to workgroups in blocks of R
-      double gs = 0.0;  // largest |value| among the other workgroups' candidates (margins only)
-      if (g == 0 && wave == 0) {
-        uint64_t o = 0;
-#pragma unroll
-        for (int q = 0; q < GQ; ++q) o = max(o, (lane + 64 * q == gwin) ? 0ull : gk[q]);
-        gs = key_value(wave_max_key(o));
-      }
       DLU_PHASE(3)
       const bool p_here = p != k && p >= base && p < base + R;
 
       // One exchange round trip for everything this wave needs: its 16 columns of the pivot row
       // (lanes 0-15), the pivot row at columns c, c+1, c+2 (lanes 16-18), and, when row p lives
       // here, the old row k at columns c, c+1, c+2 (lanes 19-21) and its 16 columns (32-47);
-      // lane 22 reads the winner's second-largest |value| (for the reported margin).
+      // lane 22 reads the winner's second-largest |value| (for the reported margin), lane 23 the
+      // winner's origin and lane 24 row k's origin.
       const uint64_t *wslot = P.slots + (static_cast<size_t>(par) * P.groups + gwin) * SW;
       const uint64_t *wrow = wslot + kSlotHeaderWords;
       const uint64_t *src = nullptr;
@@ -478,6 +543,8 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
       if (p_here && lane >= 19 && lane < 22 && c + lane - 19 < width)
         src = rk + WPV * (c + lane - 19);
       if (lane == 22) src = wslot + 2;
+      if (lane == 23) src = wslot + 4;
+      if (p_here && lane == 24) src = rk + 2 * NB;
       if (p_here && lane >= 32 && lane < 48 && col0 + lane - 32 < width)
         src = rk + WPV * (col0 + lane - 32);
       T got = T(0);
@@ -492,11 +559,9 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
       const T u1 = lane_value(got, 17), u2 = lane_value(got, 18);
       const T kc = lane_value(got, 19);
       const T k1 = lane_value(got, 20), k2 = lane_value(got, 21);
-      const double g2 = fmax(gs, static_cast<double>(lane_value(got, 22)));
+      const double w2 = static_cast<double>(lane_value(got, 22));
+      const int o_p = from_word(lane_value(got, 23)), o_k = from_word(lane_value(got, 24));
       wave_sync_lds();
-      T u[16];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) u[j] = s_u[wave][j];
       DLU_PHASE(4)
 
       // Row k is final now: it is the pivot row (the L part, the pivot and the U part).  The
@@ -507,11 +572,10 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
             s_pub[wave][0][lane];
 
       // Move the old row k into row p (only the lane holding row p), form the multipliers
-      // (column c times the reciprocal of the pivot) and apply the rank-1 update to the panel
-      // columns right of c and to the window.  Every row is updated: those at or above k hold
-      // values that are never read again, and padding rows beyond m are never written back.
+      // (column c times the reciprocal of the pivot) and update the window; the rank-1 update
+      // of the panel columns right of c becomes pending.  Every row is updated: those at or above
+      // k hold values that are never read again, and padding rows beyond m are never written back.
       const T recip = T(1) / piv;
-      const int jc = c - col0;  // this wave's column index of c (may be out of 0..15)
 #pragma unroll
       for (int r = 0; r < RPL; ++r) {
         const int row = base + 64 * r + lane;
@@ -521,26 +585,22 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
           for (int j = 0; j < 16; ++j) v[r][j] = s_k[wave][j];
           wb[r] = k1;
           wc[r] = k2;
+          if (wave == NW - 1) s_from[row - base] = o_k;
         }
         const T l = (is_p ? kc : wa[r]) * recip;
-        if (jc < 0) {  // wave-uniform: all of this wave's columns lie right of c
-#pragma unroll
-          for (int j = 0; j < 16; ++j) v[r][j] = fma(-l, u[j], v[r][j]);
-        } else if (jc < 16) {
-#pragma unroll
-          for (int j = 0; j < 16; ++j) {
-            if (j == jc) v[r][j] = l;
-            if (j > jc) v[r][j] = fma(-l, u[j], v[r][j]);
-          }
-        }
+        s_lp[wave][64 * r + lane] = l;
         wb[r] = fma(-l, u1, wb[r]);
         wc[r] = fma(-l, u2, wc[r]);
       }
+      jp = c - col0;  // may be out of 0..15
+      pend = true;
       if (g == 0 && threadIdx.x == 0) {
         s_hist[c] = p;
+        s_orig[c] = o_p;
         s_g1[c] = key_value(g1k);
-        s_g2[c] = g2;
       }
+      prev_gwin = gwin;
+      prev_w2 = w2;
 #pragma unroll
       for (int r = 0; r < RPL; ++r) {
         wa[r] = wb[r];
@@ -549,117 +609,110 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
       DLU_PHASE(5)
     }
   }
+  DLU_APPLY_PENDING()
+#undef DLU_APPLY_PENDING
   __syncthreads();  // exchange reads of the last step are checked here
   if (s_fail) goto leave;
-
-  // Write the rows not yet written back: L multipliers and the Schur columns of rows [stop, m)
-  // (rows [pb, stop) went out as they were finished).
-#pragma unroll
-  for (int r = 0; r < RPL; ++r) {
-    const int row = base + 64 * r + lane;
-    if (row >= stop && row < m) {
-#pragma unroll
-      for (int j = 0; j < 16; ++j)
-        if (col0 + j < width)
-          P.a[static_cast<size_t>(row) + static_cast<size_t>(P.pb + col0 + j) * P.lda] = v[r][j];
-    }
-  }
-  if (g == 0) {
+  {
     const int steps = stop - P.pb;
-    // Pivot history and margins of the eliminated steps (a forced step keeps the row the
-    // fallback recorded and has no margin).
-    for (int s = threadIdx.x; s < steps; s += blockDim.x) {
-      if (!(P.positioned && s == 0)) P.ipiv[P.pb + s] = s_hist[s];
-      if (P.margins)
-        P.margins[P.pb + s] =
-            (P.positioned && s == 0) ? -1.0 : (s_g1[s] - fmax(s_g2[s], 0.0)) / s_g1[s];
-    }
-    if (wave == 0) {  // tie statistics of the column steps
-      double lo = 2.0;
-      int near = 0, exact = 0;
-      for (int s0 = 0; s0 < steps; s0 += 64) {
-        const int s = s0 + lane;
-        const bool valid = s < steps && !(P.positioned && s == 0);
-        const double mg = valid ? (s_g1[s] - fmax(s_g2[s], 0.0)) / s_g1[s] : 2.0;
-        lo = fmin(lo, mg);
-        near += __popcll(__ballot(valid && mg < 1e-10));
-        exact += __popcll(__ballot(valid && mg == 0.0));
+    // Margin of the last eliminated step (its successor never ran).
+    if (g == 0 && wave == 1 && prev_gwin >= 0) {
+      constexpr int GQ = kMaxPanelGroups / 64;
+      const int cl = steps - 1;
+      uint64_t o = 0;
+#pragma unroll
+      for (int q = 0; q < GQ; ++q) {
+        const int gi = lane + 64 * q;
+        const uint64_t kq = s_ck[cl & 1][gi];
+        o = max(o, (gi < P.groups && gi != prev_gwin) ? kq : 0ull);
       }
-      min_margin = -wave_max(-lo);
-      near_ties = near;
-      exact_ties = exact;
+      const double gs = key_value(wave_max_key(o));
+      if (lane == 0) s_g2[cl] = fmax(gs, prev_w2);
     }
-    // Net row movement of steps [pb, stop) as (dst, src) pairs: rows [pb, stop) first, in order,
-    // then every other row whose content changed.  Built by wave 0 from the pivot history.
+
+    // Write the rows not yet written back: L multipliers and the Schur columns of rows [stop, m)
+    // (rows [pb, stop) went out as they were finished).
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      const int row = base + 64 * r + lane;
+      if (row >= stop && row < m) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+          if (col0 + j < width)
+            P.a[static_cast<size_t>(row) + static_cast<size_t>(P.pb + col0 + j) * P.lda] = v[r][j];
+      }
+    }
+
+    // Net row movement as (dst, src) pairs: rows [pb, stop) first, in order (workgroup 0), then
+    // every row at or below stop whose content changed (each workgroup its own, appended).
+    int *dst = P.swap_list;
+    int *src = P.swap_list + 2 * NB;
     if (wave == 0) {
-      for (int i = lane; i < NB; i += 64) s_cur[i] = P.pb + i;
-      int out_cnt = 0;
-      wave_sync_lds();
-      for (int s = 0; s < steps; ++s) {
-        const int p = s_hist[s];
-        if (p < P.pb + NB) {
-          if (lane == 0) {
-            const int t = s_cur[s];
-            s_cur[s] = s_cur[p - P.pb];
-            s_cur[p - P.pb] = t;
-          }
-        } else {
-          int found = -1;
-          for (int j0 = 0; j0 < out_cnt; j0 += 64) {
-            const int j = j0 + lane;
-            const unsigned long long hit = __ballot(j < out_cnt && s_out_row[j] == p);
-            if (hit) {
-              found = j0 + __ffsll(static_cast<long long>(hit)) - 1;
-              break;
-            }
-          }
-          if (found < 0) {
-            found = out_cnt++;
-            if (lane == 0) {
-              s_out_row[found] = p;
-              s_out_cur[found] = p;
-            }
-            wave_sync_lds();
-          }
-          if (lane == 0) {
-            const int t = s_cur[s];
-            s_cur[s] = s_out_cur[found];
-            s_out_cur[found] = t;
-          }
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) {
+        const int row = base + 64 * r + lane;
+        const int from = s_from[64 * r + lane];
+        if (row >= stop && row < m && from != row) {
+          const int at = steps + atomicAdd(&P.swap_list[4 * NB + 1], 1);
+          dst[at] = row;
+          src[at] = from;
         }
-        wave_sync_lds();
       }
-      if (lane == 0) {
-        int *dst = P.swap_list;
-        int *src = P.swap_list + 2 * NB;
-        int cnt = 0;
-        for (int i = 0; i < steps; ++i) {
-          dst[cnt] = P.pb + i;
-          src[cnt] = s_cur[i];
-          ++cnt;
+    }
+    if (g == 0) {
+      for (int i = threadIdx.x; i < steps; i += blockDim.x) {
+        dst[i] = P.pb + i;
+        src[i] = s_orig[i];
+      }
+    }
+    __threadfence();
+    __syncthreads();  // s_g2 complete; this workgroup's list entries are out
+    if (threadIdx.x == 0) {
+      // The last workgroup to finish closes the list and clears the counters for the next panel
+      // that uses this list.
+      if (atomicAdd(&P.swap_list[4 * NB + 2], 1) == P.groups - 1) {
+        __threadfence();
+        const int appended = __hip_atomic_load(&P.swap_list[4 * NB + 1], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        P.swap_list[4 * NB] = steps + appended;
+        P.swap_list[4 * NB + 1] = 0;
+        P.swap_list[4 * NB + 2] = 0;
+      }
+    }
+    if (g == 0) {
+      // Pivot history and margins of the eliminated steps (a forced step keeps the row the
+      // fallback recorded and has no margin).
+      for (int s = threadIdx.x; s < steps; s += blockDim.x) {
+        if (!(P.positioned && s == 0)) P.ipiv[P.pb + s] = s_hist[s];
+        if (P.margins)
+          P.margins[P.pb + s] =
+              (P.positioned && s == 0) ? -1.0 : (s_g1[s] - fmax(s_g2[s], 0.0)) / s_g1[s];
+      }
+      if (wave == 0) {  // tie statistics of the column steps
+        double lo = 2.0;
+        int near = 0, exact = 0;
+        for (int s0 = 0; s0 < steps; s0 += 64) {
+          const int s = s0 + lane;
+          const bool valid = s < steps && !(P.positioned && s == 0);
+          const double mg = valid ? (s_g1[s] - fmax(s_g2[s], 0.0)) / s_g1[s] : 2.0;
+          lo = fmin(lo, mg);
+          near += __popcll(__ballot(valid && mg < 1e-10));
+          exact += __popcll(__ballot(valid && mg == 0.0));
         }
-        for (int i = steps; i < NB; ++i)
-          if (s_cur[i] != P.pb + i) {
-            dst[cnt] = P.pb + i;
-            src[cnt] = s_cur[i];
-            ++cnt;
+        min_margin = -wave_max(-lo);
+        near_ties = near;
+        exact_ties = exact;
+        if (lane == 0) {
+          if (timed) {
+            DLU_PHASE(7)
+            for (int i = 0; i < 8; ++i) P.phase_ticks[i] += s_ph[i];
           }
-        for (int j = 0; j < out_cnt; ++j)
-          if (s_out_cur[j] != s_out_row[j]) {
-            dst[cnt] = s_out_row[j];
-            src[cnt] = s_out_cur[j];
-            ++cnt;
-          }
-        P.swap_list[4 * NB] = cnt;
-        if (timed) {
-          DLU_PHASE(7)
-          for (int i = 0; i < 8; ++i) P.phase_ticks[i] += ph[i];
+          P.ctl->stop = stop;
+          P.ctl->zero_column = zero_col;
+          P.ctl->near_ties += near_ties;
+          P.ctl->exact_ties += exact_ties;
+          if (min_margin < P.ctl->min_margin) P.ctl->min_margin = min_margin;
         }
-        P.ctl->stop = stop;
-        P.ctl->zero_column = zero_col;
-        P.ctl->near_ties += near_ties;
-        P.ctl->exact_ties += exact_ties;
-        if (min_margin < P.ctl->min_margin) P.ctl->min_margin = min_margin;
       }
     }
   }
