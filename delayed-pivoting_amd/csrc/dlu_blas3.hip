@@ -19,8 +19,8 @@ namespace {
 // chunk feeds the MFMA pipe.  Global loads are lane-contiguous (consecutive lanes read
 // consecutive rows of A, consecutive k of B), 16 bytes per lane when the operands allow it.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int BM, int BN, int BK, bool VEC>
-__global__ __launch_bounds__(256, 2) void gemm_sub_kernel(T *__restrict__ C, int ldc,
+template <typename T, int BM, int BN, int BK, bool VEC, int OCC>
+__global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, int ldc,
                                                          const T *__restrict__ A, int lda,
                                                          const T *__restrict__ B, int ldb,
                                                          int M, int N, int K, int tiles_m,
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(64) void swap_trsm_kernel(T *__restrict__ a, int ld
     }
 }
 
-template <typename T, int BM, int BN, int BK>
+template <typename T, int BM, int BN, int BK, int OCC>
 hipError_t launch_gemm_tile(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
                             int K, hipStream_t s) {
   const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
@@ -351,11 +351,11 @@ hipError_t launch_gemm_tile(T *c, int ldc, const T *a, int lda, const T *b, int 
                    (reinterpret_cast<uintptr_t>(a) % 16 == 0) &&
                    (reinterpret_cast<uintptr_t>(b) % 16 == 0);
   if (vec)
-    hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK, true>), dim3(tm * tn), dim3(256), 0, s, c,
-                       ldc, a, lda, b, ldb, M, N, K, tm, tn);
+    hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK, true, OCC>), dim3(tm * tn), dim3(256), 0,
+                       s, c, ldc, a, lda, b, ldb, M, N, K, tm, tn);
   else
-    hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK, false>), dim3(tm * tn), dim3(256), 0, s,
-                       c, ldc, a, lda, b, ldb, M, N, K, tm, tn);
+    hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK, false, OCC>), dim3(tm * tn), dim3(256), 0,
+                       s, c, ldc, a, lda, b, ldb, M, N, K, tm, tn);
   return hipGetLastError();
 }
 
@@ -365,9 +365,12 @@ template <typename T>
 hipError_t launch_gemm_sub(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
                            int K, hipStream_t s) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
-  if (gemm_uses_large_tiles(M, N))
-    return launch_gemm_tile<T, 128, 128, 16>(c, ldc, a, lda, b, ldb, M, N, K, s);
-  return launch_gemm_tile<T, 64, 64, 16>(c, ldc, a, lda, b, ldb, M, N, K, s);
+  // FP64: 64 x 64 tiles at four workgroups per CU outrun 128 x 128 tiles at two for every shape
+  // of this factorization (more waves in flight hide the C, A and B traffic of short K).  FP32
+  // MFMAs retire twice as fast, so large FP32 problems keep the 128 x 128 tile's data reuse.
+  if (sizeof(T) == 4 && gemm_uses_large_tiles(M, N))
+    return launch_gemm_tile<T, 128, 128, 16, 2>(c, ldc, a, lda, b, ldb, M, N, K, s);
+  return launch_gemm_tile<T, 64, 64, 16, 4>(c, ldc, a, lda, b, ldb, M, N, K, s);
 }
 
 template <typename T>

@@ -354,7 +354,7 @@ struct Run {
     if (!split || panel_cus <= 0 || c0 >= n) return n;
     const int rows = m - pb - wdt;
     const int U = std::max(1, w.cus - panel_cus), P = panel_cus;
-    const double rate = sizeof(T) == 8 ? 0.135e12 : 0.27e12;     // flop/s per CU
+    const double rate = sizeof(T) == 8 ? 0.145e12 : 0.27e12;     // flop/s per CU
     const double work = 2.0 * rows * static_cast<double>(n - c0) * wdt;
     const double t_panel = 5.2e-6 * next_wdt, t_solve = 7.5e-9 * (n - c0);
     const double t = (work / rate + P * t_panel + U * t_solve) / (P + U);
