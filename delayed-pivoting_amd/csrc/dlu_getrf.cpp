@@ -80,6 +80,7 @@ struct Workspace {
 constexpr int kMaxDevices = 64;
 constexpr int kSearchPartials = 2048;
 constexpr int kListStride = 4 * 256 + 4;  // swap_list_ints(256), enough for every nb
+constexpr int kLists = 4;  // panels j and j+4 share a list (a pair's far update reads two)
 
 // Exchange buffers, sized for the largest panel width (256) and workgroup count.
 constexpr size_t slot_bytes() {
@@ -103,7 +104,7 @@ hipError_t prepare(Workspace &w, int device, int steps) {
     if ((e = hipMemset(w.d_slots, 0, slot_bytes())) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_rowk, rowk_bytes())) != hipSuccess) return e;
     if ((e = hipMemset(w.d_rowk, 0, rowk_bytes())) != hipSuccess) return e;
-    if ((e = hipMalloc(&w.d_lists, 2 * kListStride * sizeof(int))) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_lists, kLists * kListStride * sizeof(int))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_pv, kSearchPartials * sizeof(double))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_po, kSearchPartials * sizeof(unsigned long long))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_phase, 8 * sizeof(unsigned long long))) != hipSuccess) return e;
@@ -184,8 +185,9 @@ struct Run {
   int panels = 0;
   int max_groups = kMaxPanelGroups;  // workgroups the panel stream can hold at once
   int forced_rpl = 0;                // rows per lane requested by the caller (0: automatic)
-  int merge_rows = 1 << 30;          // pair panels while at least this many rows remain
+  int merge_rows = 3000;             // pair panels while at least this many rows remain
   bool split = true;                 // tail of the far GEMM on the panel stream's CUs
+  int forced_split = 0;              // > 0: that many far columns always go to the panel stream
   int panel_cus = 0;                 // CUs of the panel stream (0: no partition)
   hipStream_t ps = nullptr, us = nullptr;  // internal panel / update streams
 
@@ -311,31 +313,71 @@ struct Run {
     return e;
   }
 
-  // Two consecutive full panels share one trailing update.  After the first panel (pb0) the
-  // columns beyond the next panel get only its interchange and U block row; their GEMM waits.
-  // After the second panel (pb1 = pb0 + nb) those columns take the second panel's interchange,
-  // the first panel's update of the second panel's rows, the second panel's U block row, and
-  // then a single GEMM with K = 2 nb for all rows below (interchanges commute with the deferred
-  // update once the first panel's L is interchanged too).
-  struct Pending {
-    bool valid = false;
-    int pb = 0;  // first panel of the pair
-    int fe = 0;  // first column still owed the first panel's GEMM
+  // Panel pairs.  While the trailing GEMM dominates, two consecutive full panels P0 (rows and
+  // columns from b0) and P1 (from b1 = b0 + nb) share one K = 2 nb update.  After P0 only the
+  // next panel's columns are brought up to date; the columns beyond wait.  After P1, for a
+  // column range: P0's interchange and U block row (TRSM), P1's interchange, P0's update of
+  // P1's rows, P1's U block row, and one GEMM with K = 2 nb for every row below (interchanges
+  // commute with the deferred update once P0's L columns carry P1's interchange too).  The
+  // next panel's block goes first, then the far columns' left half; their right half follows the
+  // next step's near update, so on the in-order update stream no near update ever waits for more
+  // than half of a pair's far work.
+  struct Pair {
+    bool open = false;  // P0 done, its far update owed
+    int b0 = 0, fe = 0;  // P0's first row/column; first column still owed P0's update
+    const int *l0 = nullptr;
   };
-  hipError_t merged_update(const Pending &pd, int pb1, int c0, int c1, const int *list,
-                           hipStream_t st) {
-    if (c1 <= c0) return hipSuccess;
-    hipError_t e = launch_row_interchange(a, lda, c0, c1, list, nb, st);
-    if (e == hipSuccess) e = gemm(pb1, c0, pd.pb, pd.pb, nb, c1 - c0, nb, st);
-    if (e == hipSuccess) e = solve(pb1, nb, c0, c1, nullptr, st);
-    if (e == hipSuccess) e = gemm(pb1 + nb, c0, pd.pb, pd.pb, m - pb1 - nb, c1 - c0, 2 * nb, st);
+  Pair pair;
+  struct Rest {  // the right half of a closed pair's far update, not yet queued
+    bool pending = false;
+    int b0 = 0, c0 = 0;
+    const int *l0 = nullptr, *l1 = nullptr;
+  };
+  Rest rest;
+  // Columns [c0, c1) up to (but without) the K = 2 nb GEMM.
+  hipError_t pair_prepare(int b0, const int *l0, const int *l1, int c0, int c1, hipStream_t st) {
+    const int b1 = b0 + nb;
+    hipError_t e = solve(b0, nb, c0, c1, l0, st);
+    if (e == hipSuccess) e = launch_row_interchange(a, lda, c0, c1, l1, nb, st);
+    if (e == hipSuccess) e = gemm(b1, c0, b0, b0, nb, c1 - c0, nb, st);
+    if (e == hipSuccess) e = solve(b1, nb, c0, c1, nullptr, st);
     return e;
   }
-  // The deferred GEMM on its own, before the next panel's interchanges reach those columns.
-  hipError_t flush(Pending &pd, hipStream_t st) {
-    if (!pd.valid) return hipSuccess;
-    pd.valid = false;
-    return gemm(pd.pb + nb, pd.fe, pd.pb, pd.pb, m - pd.pb - nb, n - pd.fe, nb, st);
+  hipError_t pair_update(int b0, const int *l0, const int *l1, int c0, int c1, hipStream_t st) {
+    if (c1 <= c0) return hipSuccess;
+    hipError_t e = pair_prepare(b0, l0, l1, c0, c1, st);
+    if (e == hipSuccess) e = gemm(b0 + 2 * nb, c0, b0, b0, m - b0 - 2 * nb, c1 - c0, 2 * nb, st);
+    return e;
+  }
+  // Queue the pending right half, then the pair's left interchanges.  With next_wdt > 0 (the
+  // width of the panel that runs meanwhile, when nothing else of this step touches far columns)
+  // its GEMM's last columns may go to the panel stream.
+  hipError_t queue_rest(int next_wdt) {
+    if (!rest.pending) return hipSuccess;
+    rest.pending = false;
+    const int b0 = rest.b0, c0 = rest.c0, r0 = b0 + 2 * nb;
+    hipError_t e = hipSuccess;
+    if (c0 < n) {
+      if ((e = wait_tail(us)) != hipSuccess) return e;
+      if ((e = pair_prepare(b0, rest.l0, rest.l1, c0, n, us)) != hipSuccess) return e;
+      const int cb = next_wdt > 0 ? tail_split(m - r0, 2 * nb, c0, next_wdt) : n;
+      if (cb < n) {
+        if ((e = hipEventRecord(w.ev_solved, us)) != hipSuccess) return e;
+        tail = Tail{true, false, r0, b0, 2 * nb, cb};
+      }
+      if ((e = gemm(r0, c0, b0, b0, m - r0, cb - c0, 2 * nb, us)) != hipSuccess) return e;
+    }
+    if ((e = launch_row_interchange(a, lda, 0, b0, rest.l0, nb, us)) != hipSuccess) return e;
+    return launch_row_interchange(a, lda, 0, b0, rest.l1, nb, us);
+  }
+  // P0's owed update on its own (the pair could not be completed).
+  hipError_t close_pair(hipStream_t st) {
+    if (!pair.open) return hipSuccess;
+    pair.open = false;
+    hipError_t e = wait_tail(st);
+    if (e == hipSuccess) e = update(pair.b0, nb, pair.fe, n, pair.l0, st);
+    if (e == hipSuccess) e = launch_row_interchange(a, lda, 0, pair.b0, pair.l0, nb, st);
+    return e;
   }
 
   // Far-update tail on the panel stream.  While the trailing GEMM outlasts the next panel, the
@@ -347,16 +389,22 @@ struct Run {
   struct Tail {
     bool queued = false;   // a tail GEMM is waiting to be launched behind the next panel
     bool pending = false;  // a launched tail GEMM may still run (ev_tail marks its end)
-    int pb = 0, wdt = 0, cb = 0;
+    int r0 = 0, kc = 0, K = 0, cb = 0;  // A(r0:m, cb:n) -= A(r0:m, kc:kc+K) A(kc:kc+K, cb:n)
   };
   Tail tail;
-  int tail_split(int pb, int wdt, int c0, int next_wdt) const {
-    if (!split || panel_cus <= 0 || c0 >= n) return n;
-    const int rows = m - pb - wdt;
+  // First column of the tail for a GEMM over rows [m - rows, m), columns [c0, n), depth wdt.
+  int tail_split(int rows, int wdt, int c0, int next_wdt) const {
+    if (!split || c0 >= n) return n;
+    if (forced_split > 0) {
+      const int cols = std::min(forced_split, n - c0 - nb);
+      return cols > 0 ? n - cols : n;
+    }
+    if (panel_cus <= 0) return n;
     const int U = std::max(1, w.cus - panel_cus), P = panel_cus;
-    const double rate = sizeof(T) == 8 ? 0.145e12 : 0.27e12;     // flop/s per CU
+    // Per-CU rate of the trailing GEMM (measured: deeper K reuses C better), panel step latency.
+    const double rate = (sizeof(T) == 8 ? 0.145e12 : 0.27e12) * (wdt > nb ? 1.3 : 1.0);
     const double work = 2.0 * rows * static_cast<double>(n - c0) * wdt;
-    const double t_panel = 5.2e-6 * next_wdt, t_solve = 7.5e-9 * (n - c0);
+    const double t_panel = 4.4e-6 * next_wdt, t_solve = 7.5e-9 * (n - c0);
     const double t = (work / rate + P * t_panel + U * t_solve) / (P + U);
     const double w_tail = P * rate * (t - t_panel);
     if (w_tail <= 0.0 || rows <= 0) return n;
@@ -375,8 +423,7 @@ struct Run {
     tail.queued = false;
     hipError_t e = hipStreamWaitEvent(ps, w.ev_solved, 0);
     if (e == hipSuccess)
-      e = gemm(tail.pb + tail.wdt, tail.cb, tail.pb, tail.pb, m - tail.pb - tail.wdt, n - tail.cb,
-               tail.wdt, ps);
+      e = gemm(tail.r0, tail.cb, tail.kc, tail.kc, m - tail.r0, n - tail.cb, tail.K, ps);
     if (e == hipSuccess) e = hipEventRecord(w.ev_tail, ps);
     if (e == hipSuccess) tail.pending = true;
     return e;
@@ -393,75 +440,69 @@ struct Run {
     int pb = 0, parity = 0;
     int pe = std::min(pb + nb, steps);
     if ((e = panel(pb, pe, 0, parity)) != hipSuccess) return e;
-    Pending pend;
     for (;;) {
       if ((e = hipEventSynchronize(w.ev_ctl)) != hipSuccess) return e;
       const Control c = *w.h_ctl;
       if (c.error != 0) return hipErrorLaunchFailure;
-      if (c.deferred) {
-        if ((e = hipStreamWaitEvent(us, w.ev_ctl, 0)) != hipSuccess) return e;
-        if ((e = wait_tail(us)) != hipSuccess) return e;
-        if ((e = flush(pend, us)) != hipSuccess) return e;
-        *info = c.stop;
-        break;
-      }
       const int stop = c.stop;
       const int wdt = stop - pb;
       const int *list = w.d_lists + parity * kListStride;
+      const int pe2 = std::min(pe + nb, steps);
+      const bool full = !c.deferred && !c.zero_column && wdt == nb && pe - pb == nb;
       if ((e = hipStreamWaitEvent(us, w.ev_ctl, 0)) != hipSuccess) return e;
-      const bool full = !c.zero_column && wdt == nb && pe - pb == nb;
+      const bool second = pair.open && full && pe < steps && lookahead;
+      if (!second && (e = close_pair(us)) != hipSuccess) return e;
+      if (c.deferred) {
+        if ((e = queue_rest(0)) != hipSuccess) return e;
+        if ((e = wait_tail(us)) != hipSuccess) return e;
+        *info = c.stop;
+        break;
+      }
       if (!c.zero_column) {
-        const int pe2 = std::min(pe + nb, steps);
-        bool merged = false;
-        const bool plain = !pend.valid && pe < steps && lookahead &&
-                           !(full && pe2 < n && m - pe2 >= merge_rows);
-        // Everything but the plain look-ahead step touches far columns before anything else.
-        if (!plain && (e = wait_tail(us)) != hipSuccess) return e;
-        if (pend.valid && full && pe < steps && lookahead) {
-          // Second panel of a pair: the next panel's columns first, then the far columns.
-          if ((e = launch_row_interchange(a, lda, pend.pb, pb, list, nb, us)) != hipSuccess)
-            return e;
-          if ((e = merged_update(pend, pb, pe, pe2, list, us)) != hipSuccess) return e;
+        if (second) {
+          // Second panel of a pair: P0's L columns take P1's interchange, then the next panel's
+          // block, then the left half of the far columns (it holds the block after next); the
+          // right half is queued after the next step's near update.
+          const int b0 = pair.b0;
+          pair.open = false;
+          if ((e = queue_rest(0)) != hipSuccess) return e;  // (none pending: a pair just opened)
+          if ((e = wait_tail(us)) != hipSuccess) return e;
+          if ((e = launch_row_interchange(a, lda, b0, pb, list, nb, us)) != hipSuccess) return e;
+          if ((e = pair_update(b0, pair.l0, list, pe, pe2, us)) != hipSuccess) return e;
           if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
-          if ((e = merged_update(pend, pb, pe2, n, list, us)) != hipSuccess) return e;
-          if ((e = launch_row_interchange(a, lda, 0, pend.pb, list, nb, us)) != hipSuccess)
-            return e;
-          pend.valid = false;
-          merged = true;
-        } else if ((e = flush(pend, us)) != hipSuccess) {
-          return e;
-        }
-        if (merged) {
-          // done: the next panel's columns are current and ev_next is recorded
+          const int half = std::max(nb, (n - pe2) / 2 / nb * nb);
+          const int cm = std::min(n, pe2 + half);
+          if ((e = pair_update(b0, pair.l0, list, pe2, cm, us)) != hipSuccess) return e;
+          rest = Rest{true, b0, cm, pair.l0, list};
         } else if (pe < steps && lookahead && full && pe2 < n && m - pe2 >= merge_rows) {
-          // First panel of a pair: full update of the next panel's columns; the far columns get
-          // the interchange and U block row now and their GEMM after the next panel.
+          // First panel of a pair: only the next panel's columns now (and the previous pair's
+          // right half, which the next panel's block does not touch).
           if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
           if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
-          if ((e = solve(pb, wdt, pe2, n, list, us)) != hipSuccess) return e;
-          if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
-          pend.valid = true;
-          pend.pb = pb;
-          pend.fe = pe2;
+          if ((e = queue_rest(pe2 - pe)) != hipSuccess) return e;
+          pair.open = true;
+          pair.b0 = pb;
+          pair.fe = pe2;
+          pair.l0 = list;
         } else if (pe < steps && lookahead) {
           // The next panel's columns first, then the far columns while the next panel runs
           // (their last columns on the panel stream behind it when the GEMM outlasts it).
           if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
           if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
+          if ((e = queue_rest(0)) != hipSuccess) return e;
           if ((e = wait_tail(us)) != hipSuccess) return e;
           if ((e = solve(pb, wdt, pe2, n, list, us)) != hipSuccess) return e;
-          const int cb = tail_split(pb, wdt, pe2, pe2 - pe);
+          const int cb = tail_split(m - pb - wdt, wdt, pe2, pe2 - pe);
           if (cb < n) {
             if ((e = hipEventRecord(w.ev_solved, us)) != hipSuccess) return e;
-            tail.queued = true;
-            tail.pb = pb;
-            tail.wdt = wdt;
-            tail.cb = cb;
+            tail = Tail{true, false, pb + wdt, pb, wdt, cb};
           }
           if ((e = gemm(pb + wdt, pe2, pb, pb, m - pb - wdt, cb - pe2, wdt, us)) != hipSuccess)
             return e;
           if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
         } else {
+          if ((e = queue_rest(0)) != hipSuccess) return e;
+          if ((e = wait_tail(us)) != hipSuccess) return e;
           if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
           if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
           if (pe >= steps) {
@@ -475,8 +516,8 @@ struct Run {
       } else {
         // Zero column at `stop`: bring the far columns up to date with the accepted prefix
         // (delayed_dgetrf.cpp:183-186), search the whole active block, interchange, restart.
+        if ((e = queue_rest(0)) != hipSuccess) return e;
         if ((e = wait_tail(us)) != hipSuccess) return e;
-        if ((e = flush(pend, us)) != hipSuccess) return e;
         if (wdt > 0) {
           if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
           if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
@@ -491,7 +532,7 @@ struct Run {
         pb = stop;
         pe = std::min(pb + nb, steps);
       }
-      parity ^= 1;
+      parity = (parity + 1) % kLists;
       if ((e = hipStreamWaitEvent(ps, w.ev_next, 0)) != hipSuccess) return e;
       if ((e = panel(pb, pe, c.zero_column ? 1 : 0, parity)) != hipSuccess) return e;
       if ((e = launch_tail()) != hipSuccess) return e;
@@ -566,6 +607,7 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   }
   if (opt && opt->merge_rows != 0) run.merge_rows = opt->merge_rows < 0 ? (1 << 30) : opt->merge_rows;
   run.split = !(opt && opt->split_update < 0);
+  if (opt && opt->split_update > 0) run.forced_split = opt->split_update;
   {
     // CUs for the panel stream: enough for one resident workgroup per panel row block, at
     // least 16 and at most half the device (opt->panel_cus > 0 forces a size, < 0 disables).
@@ -606,7 +648,7 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   // this also recovers from an aborted call).
   if ((e = hipMemcpyAsync(w.d_ctl, w.h_ctl, sizeof(Control), hipMemcpyHostToDevice, stream)) ==
           hipSuccess &&
-      (e = hipMemsetAsync(w.d_lists, 0, 2 * kListStride * sizeof(int), stream)) == hipSuccess &&
+      (e = hipMemsetAsync(w.d_lists, 0, kLists * kListStride * sizeof(int), stream)) == hipSuccess &&
       (e = launch_init_steps(w.d_ipiv, w.d_jpiv, w.d_margins, steps, stream)) == hipSuccess) {
     int out_info = steps;
     e = run.factor(&out_info, lookahead);

@@ -27,11 +27,11 @@ def _torch():
     return torch
 
 
-def gpu_factor(a, nb=None, lda=None, lookahead=None, margins=True):
+def gpu_factor(a, nb=None, lda=None, lookahead=None, margins=True, sched=None):
     torch = _torch()
     m, n = a.shape
     t, lda = dlu.to_device(a, lda)
-    opts = {}
+    opts = dict(sched or {})
     if nb:
         opts["nb"] = nb
     if lookahead is not None:
@@ -237,3 +237,30 @@ def test_large_normal_parity(n):
         assert marg[k] < 1e-10, f"step {k} differs with margin {marg[k]}"
     assert np.array_equal(jpiv, jp_c)
     assert np.linalg.norm(lu - lu_c) <= FACTOR_RTOL * np.linalg.norm(lu_c)
+
+
+# Update schedules: panel pairs sharing a K = 2 nb far update on a third stream (merge_rows=1
+# pairs every eligible panel), and the far GEMM's last columns moved to the panel stream
+# (split_update > 0 forces it at these sizes).  Every schedule must give the oracle's result.
+SCHEDULES = [{"merge_rows": 1}, {"merge_rows": 1, "split_update": 384}, {"split_update": 384},
+             {"merge_rows": 1, "split_update": -1}]
+
+
+@pytest.mark.parametrize("sched", SCHEDULES, ids=lambda d: ",".join(f"{k}={v}" for k, v in d.items()))
+@pytest.mark.parametrize("case", ["normal", "rect_tall", "rect_wide", "stress", "zero_cols"])
+def test_schedules_match_oracle(sched, case):
+    if case == "normal":
+        a = workloads.normal(1300, 1300, 404)
+    elif case == "rect_tall":
+        a = workloads.normal(1500, 900, 405)
+    elif case == "rect_wide":
+        a = workloads.normal(900, 1400, 406)
+    elif case == "stress":
+        a, zc, _, _ = workloads.delay_stress(1024, 7)
+    else:
+        a = workloads.normal(1200, 1200, 407)
+        a[:, [300, 301, 650, 1000]] = 0.0  # breaks pairs mid-way
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a, nb=128, sched=sched)
+    check_parity(a, lu, ipiv, jpiv, info, marg)
+    ref = gpu_factor(a, nb=128)
+    assert np.array_equal(ipiv, ref[1]) and np.array_equal(jpiv, ref[2]) and info == ref[3]
