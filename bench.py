@@ -233,7 +233,7 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--nb", type=int, default=0, help="panel width (0: library default)")
     ap.add_argument("--seed", type=int, default=16384)
-    ap.add_argument("--panel-cus", type=int, default=-1,
+    ap.add_argument("--panel-cus", type=int, default=0,
                     help="CUs reserved for panel kernels (0 auto, -1 none)")
     ap.add_argument("--cpu-sample", type=int, default=8192)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -245,6 +245,9 @@ def main():
     ap.add_argument("--mixed-n", type=int, default=16384)
     ap.add_argument("--serve-requests", type=int, default=1500,
                     help="requests of the serving stream measured per precision mode")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB device to device three times (the reference kernel "
+                         "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -253,6 +256,14 @@ def main():
 
     import numpy as np
     import torch
+    if args.calibrate_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": (1 << 27) * 8, "copies": 3}), flush=True)
+        return 0
     dlu = _load("dlu_amd", os.path.join(PKG, "dlu.py"))
     dlu.lib()
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -301,7 +312,7 @@ def main():
 
     # One profiled factorization: per-launch HIP events on each kernel's own stream.
     _, _, _, prep = step(profile=True)
-    # dominant kernel: the 128 x 128-tile trailing-update GEMM
+    # dominant kernel: the trailing-update GEMM (launches over at least 512 128 x 128 blocks of C)
     upd_launch_s = prep["large_update_seconds"] / max(1, prep["large_update_launches"])
     upd_flops = prep["large_update_flops"] / max(1, prep["large_update_launches"])
     achieved = upd_flops / upd_launch_s / 1e12 if upd_launch_s > 0 else 0.0
@@ -310,10 +321,12 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("gemm_sub_kernel_bytes_per_launch")
+    alg_bytes = prep.get("large_update_bytes", 0.0) / max(1, prep["large_update_launches"])
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                "traffic": traffic,
-                "kernel": "gemm_sub_kernel<128,128,16> (A22 -= L21*U12, FP64 MFMA)",
+                "traffic": traffic, "algorithmic_bytes_per_launch": round(alg_bytes),
+                "kernel": "gemm_sub_kernel<double,64,64,16,true,4> (A22 -= L21*U12, FP64 MFMA, "
+                          "K = 128 single panels and K = 256 panel pairs)",
                 "launches": prep["large_update_launches"],
                 "mean_launch_ms": round(upd_launch_s * 1e3, 4),
                 "flops_per_launch": upd_flops}

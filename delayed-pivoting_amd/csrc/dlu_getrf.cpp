@@ -194,11 +194,11 @@ struct Run {
   // Opt-in launch timing: an event pair around each launch, on the launch's own stream.
   struct Timed {
     hipEvent_t begin, end;
-    double flops;
+    double flops, bytes;  // algorithmic flops and memory bytes of the launch
   };
   std::vector<Timed> t_update, t_update_big, t_panel, t_solve;
-  hipError_t mark(std::vector<Timed> &v, hipStream_t st, double flops) {
-    Timed t{nullptr, nullptr, flops};
+  hipError_t mark(std::vector<Timed> &v, hipStream_t st, double flops, double bytes = 0.0) {
+    Timed t{nullptr, nullptr, flops, bytes};
     hipError_t e = hipEventCreate(&t.begin);
     if (e == hipSuccess) e = hipEventCreate(&t.end);
     if (e == hipSuccess) e = hipEventRecord(t.begin, st);
@@ -206,14 +206,16 @@ struct Run {
     return e;
   }
   hipError_t close(std::vector<Timed> &v, hipStream_t st) { return hipEventRecord(v.back().end, st); }
-  static double total(const std::vector<Timed> &v, double *flops) {
-    double sec = 0.0, fl = 0.0;
+  static double total(const std::vector<Timed> &v, double *flops, double *bytes = nullptr) {
+    double sec = 0.0, fl = 0.0, by = 0.0;
     for (const Timed &t : v) {
       float ms = 0.0f;
       if (t.begin && t.end && hipEventElapsedTime(&ms, t.begin, t.end) == hipSuccess) sec += ms * 1e-3;
       fl += t.flops;
+      by += t.bytes;
     }
     if (flops) *flops = fl;
+    if (bytes) *bytes = by;
     return sec;
   }
   void release() {
@@ -285,8 +287,11 @@ struct Run {
   hipError_t gemm(int r0, int c0, int kc, int kr, int M, int N, int K, hipStream_t st) {
     if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
     const double flops = 2.0 * M * static_cast<double>(N) * K;
+    // C read and written once, A and B read once: the least memory traffic of the launch.
+    const double bytes =
+        sizeof(T) * (2.0 * M * static_cast<double>(N) + static_cast<double>(K) * (M + N));
     std::vector<Timed> &tv = gemm_uses_large_tiles(M, N) ? t_update_big : t_update;
-    hipError_t e = profile ? mark(tv, st, flops) : hipSuccess;
+    hipError_t e = profile ? mark(tv, st, flops, bytes) : hipSuccess;
     if (e == hipSuccess)
       e = launch_gemm_sub<T>(at(a, lda, r0, c0), lda, at(a, lda, r0, kc), lda, at(a, lda, kr, c0),
                              lda, M, N, K, st);
@@ -609,15 +614,15 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   run.split = !(opt && opt->split_update < 0);
   if (opt && opt->split_update > 0) run.forced_split = opt->split_update;
   {
-    // CUs for the panel stream: enough for one resident workgroup per panel row block, at
-    // least 16 and at most half the device (opt->panel_cus > 0 forces a size, < 0 disables).
+    // CUs reserved for the panel stream: none by default (measured: the trailing update gains
+    // more from the whole device than the panels lose by waiting for update workgroups to drain
+    // from a CU); opt->panel_cus > 0 reserves that many, which also lets the far GEMM's tail
+    // run on them behind each panel.
     int groups = 0, rpl = 0;
     run.max_groups = std::min(kMaxPanelGroups, w.cus);
     run.panel_shape(0, groups, rpl);
-    int want = ((groups + 15) / 16) * 16;
-    want = std::max(16, std::min(want, w.cus / 2));
+    int want = 0;
     if (opt && opt->panel_cus > 0) want = std::min(opt->panel_cus, w.cus);
-    if (opt && opt->panel_cus < 0) want = 0;
     if (groups > want && want > 0) want = 0;  // too tall for a partition: share the device
     Workspace::Streams *st = nullptr;
     if ((e = stream_pair(w, want, st)) != hipSuccess) {
@@ -680,7 +685,7 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
     if (run.profile) {
       rep->large_update_launches = static_cast<int>(run.t_update_big.size());
       rep->large_update_seconds =
-          Run<T>::total(run.t_update_big, &rep->large_update_flops);
+          Run<T>::total(run.t_update_big, &rep->large_update_flops, &rep->large_update_bytes);
       rep->update_launches = static_cast<int>(run.t_update.size()) + rep->large_update_launches;
       rep->update_seconds = Run<T>::total(run.t_update, &rep->update_flops) + rep->large_update_seconds;
       rep->update_flops += rep->large_update_flops;

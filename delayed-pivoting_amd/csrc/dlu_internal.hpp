@@ -141,11 +141,24 @@ template <typename T>
 hipError_t launch_gather_rows(const T *src, int lds, T *dst, int ldd, const int *perm, int n,
                               int nrhs, hipStream_t s);
 
-// Rows [r0, r0+w) of X (w <= 128) := T^{-1} X(rows), T the diagonal block of the factor at
-// (r0, r0): unit lower (upper = false) or non-unit upper (upper = true).
+// Device state of the one-launch triangular solves: per-block completion flags (holding the
+// epoch of the launch that finished the block), a ticket counter that only grows (each launch
+// takes ticket_base .. ticket_base + blocks - 1) and an error word set by an expired wait.
+struct ChainState {
+  int *flags;
+  unsigned long long *ticket;
+  unsigned long long ticket_base;
+  int epoch;
+  long long spin_limit;  // 100 MHz ticks
+  int *err;
+};
+
+// X := T^{-1} X in one launch, T the unit lower (upper = false) or non-unit upper triangle of
+// the n x n factor; X has nrhs <= 16 columns and ldx >= n rounded up to a multiple of 64 (the
+// padding rows are scratch).  Consumes ceil(n / 64) tickets.
 template <typename T>
-hipError_t launch_block_trsv(const T *lu, int lda, int r0, int w, T *x, int ldx, int nrhs,
-                             bool upper, hipStream_t s);
+hipError_t launch_trsv_chain(const T *lu, int lda, int n, T *x, int ldx, int nrhs, bool upper,
+                             const ChainState &cs, hipStream_t s);
 
 // R = B - A X in FP64 (A n x n, X and B n x nrhs).
 hipError_t launch_residual(const double *a, int lda, const double *x, int ldx, const double *b,

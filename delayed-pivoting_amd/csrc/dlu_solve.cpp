@@ -15,13 +15,18 @@
 namespace dlu {
 namespace {
 
-constexpr int kSolveBlock = 128;  // rows per diagonal block of the triangular solves
+constexpr int kChainRows = 64;         // rows per block of the one-launch triangular solves
+constexpr int kChainRhs = 16;          // right-hand sides per triangular-solve launch
+constexpr long long kChainSpin = 200000000ll;  // bounded waits: 2 s of the 100 MHz clock
 
 // Device buffers of the solve side, grown on demand and kept per device.
 struct SolveCache {
-  enum { A32, A64, V64A, V64B, V32A, V32B, NORMW, NORMS, PERM, COUNT };
+  enum { A32, A64, V64A, V64B, V32A, V32B, NORMW, NORMS, PERM, WORK, CHAIN, COUNT };
   void *buf[COUNT] = {};
   size_t cap[COUNT] = {};
+  int epoch = 0;                       // last epoch handed to a triangular-solve launch
+  unsigned long long ticket_base = 0;  // tickets taken so far from the CHAIN counter
+  bool chain_fresh = false;            // CHAIN was (re)allocated and holds no zeros yet
   hipError_t get(int i, size_t bytes, void **out) {
     if (bytes == 0) bytes = 16;
     if (cap[i] < bytes) {
@@ -31,9 +36,47 @@ struct SolveCache {
       hipError_t e = hipMalloc(&buf[i], bytes);
       if (e != hipSuccess) return e;
       cap[i] = bytes;
+      if (i == CHAIN) chain_fresh = true;
     }
     *out = buf[i];
     return hipSuccess;
+  }
+  // Chain state for one triangular-solve launch over `blocks` row blocks, on stream s.  CHAIN
+  // layout: the ticket counter (8 bytes), the error word (4 bytes, then padding), the per-block
+  // flags.  A fresh area is cleared on s itself, so the clearing is ordered before the launch
+  // whatever kind of stream s is.
+  hipError_t chain(int blocks, ChainState &cs, hipStream_t s) {
+    void *p = nullptr;
+    hipError_t e = get(CHAIN, 16 + sizeof(int) * static_cast<size_t>(blocks), &p);
+    if (e != hipSuccess) return e;
+    if (chain_fresh || epoch == 0x7FFFFFFF) {  // new area, or flags would repeat an epoch
+      if ((e = hipMemsetAsync(p, 0, cap[CHAIN], s)) != hipSuccess) return e;
+      chain_fresh = false;
+      epoch = 0;
+      ticket_base = 0;
+    }
+    cs.ticket = static_cast<unsigned long long *>(p);
+    cs.err = reinterpret_cast<int *>(static_cast<char *>(p) + 8);
+    cs.flags = reinterpret_cast<int *>(static_cast<char *>(p) + 16);
+    cs.ticket_base = ticket_base;
+    cs.epoch = ++epoch;
+    cs.spin_limit = kChainSpin;
+    ticket_base += static_cast<unsigned long long>(blocks);
+    return hipSuccess;
+  }
+  // After the stream has drained: did a triangular-solve wait expire?  Clears the flag.
+  hipError_t chain_error(hipStream_t s, bool *failed) {
+    *failed = false;
+    if (!buf[CHAIN]) return hipSuccess;
+    int h = 0;
+    int *d_err = reinterpret_cast<int *>(static_cast<char *>(buf[CHAIN]) + 8);
+    hipError_t e = hipMemcpyAsync(&h, d_err, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess && h != 0) {
+      *failed = true;
+      chain_fresh = true;  // the counter may have drifted from ticket_base: clear before reuse
+    }
+    return e;
   }
 };
 // Per host thread (the serving workers and callers each keep their own scratch).
@@ -55,28 +98,29 @@ bool perms_valid(int n, const int *ipiv, const int *jpiv) {
   return true;
 }
 
-// X = Q U^{-1} L^{-1} P B, in place in b; work holds n x nrhs elements (leading dimension n).
+// X = Q U^{-1} L^{-1} P B, in place in b: a gather, one launch per triangle and per 16
+// right-hand sides, a gather.  Scratch (the permuted right-hand sides, rows padded to a
+// multiple of 64, and the chain state) comes from the cache.
 template <typename T>
 hipError_t getrs_core(int n, int nrhs, const T *lu, int lda, const int *d_prow, const int *d_pcol,
-                      T *b, int ldb, T *work, hipStream_t s) {
-  hipError_t e = launch_gather_rows<T>(b, ldb, work, n, d_prow, n, nrhs, s);
-  for (int r0 = 0; e == hipSuccess && r0 < n; r0 += kSolveBlock) {
-    const int w = std::min(kSolveBlock, n - r0);
-    e = launch_block_trsv<T>(lu, lda, r0, w, work, n, nrhs, false, s);
-    if (e == hipSuccess && r0 + w < n)
-      e = launch_gemm_sub<T>(work + r0 + w, n,
-                             lu + static_cast<size_t>(r0 + w) + static_cast<size_t>(r0) * lda, lda,
-                             work + r0, n, n - r0 - w, nrhs, w, s);
+                      T *b, int ldb, SolveCache &c, hipStream_t s) {
+  const int npad = (n + kChainRows - 1) / kChainRows * kChainRows;
+  const int blocks = npad / kChainRows;
+  void *pw = nullptr;
+  hipError_t e = c.get(SolveCache::WORK, sizeof(T) * static_cast<size_t>(npad) * nrhs, &pw);
+  if (e != hipSuccess) return e;
+  T *work = static_cast<T *>(pw);
+  e = launch_gather_rows<T>(b, ldb, work, npad, d_prow, n, nrhs, s);
+  for (int c0 = 0; e == hipSuccess && c0 < nrhs; c0 += kChainRhs) {
+    const int nc = std::min(kChainRhs, nrhs - c0);
+    T *xc = work + static_cast<size_t>(c0) * npad;
+    ChainState cs{};
+    if ((e = c.chain(blocks, cs, s)) == hipSuccess)
+      e = launch_trsv_chain<T>(lu, lda, n, xc, npad, nc, false, cs, s);
+    if (e == hipSuccess && (e = c.chain(blocks, cs, s)) == hipSuccess)
+      e = launch_trsv_chain<T>(lu, lda, n, xc, npad, nc, true, cs, s);
   }
-  const int last = ((n - 1) / kSolveBlock) * kSolveBlock;
-  for (int r0 = last; e == hipSuccess && r0 >= 0; r0 -= kSolveBlock) {
-    const int w = std::min(kSolveBlock, n - r0);
-    e = launch_block_trsv<T>(lu, lda, r0, w, work, n, nrhs, true, s);
-    if (e == hipSuccess && r0 > 0)
-      e = launch_gemm_sub<T>(work, n, lu + static_cast<size_t>(r0) * lda, lda, work + r0, n, r0,
-                             nrhs, w, s);
-  }
-  if (e == hipSuccess) e = launch_gather_rows<T>(work, n, b, ldb, d_pcol, n, nrhs, s);
+  if (e == hipSuccess) e = launch_gather_rows<T>(work, npad, b, ldb, d_pcol, n, nrhs, s);
   return e;
 }
 
@@ -98,17 +142,17 @@ dlu_status getrs_device(int n, int nrhs, const T *d_lu, int lda, const int *ipiv
   SolveCache &c = t_cache[device];
   std::vector<int> perms;
   build_perms(n, ipiv, jpiv, perms);
-  void *pw = nullptr, *pp = nullptr;
-  hipError_t e = c.get(sizeof(T) == 4 ? SolveCache::V32A : SolveCache::V64A,
-                       sizeof(T) * static_cast<size_t>(n) * nrhs, &pw);
-  if (e == hipSuccess) e = c.get(SolveCache::PERM, 2 * sizeof(int) * static_cast<size_t>(n), &pp);
+  void *pp = nullptr;
+  hipError_t e = c.get(SolveCache::PERM, 2 * sizeof(int) * static_cast<size_t>(n), &pp);
   if (e == hipSuccess)
     e = hipMemcpyAsync(pp, perms.data(), 2 * sizeof(int) * static_cast<size_t>(n),
                        hipMemcpyHostToDevice, s);
   const int *d_perm = static_cast<const int *>(pp);
   if (e == hipSuccess)
-    e = getrs_core<T>(n, nrhs, d_lu, lda, d_perm, d_perm + n, d_b, ldb, static_cast<T *>(pw), s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
+    e = getrs_core<T>(n, nrhs, d_lu, lda, d_perm, d_perm + n, d_b, ldb, c, s);
+  bool failed = false;
+  if (e == hipSuccess) e = c.chain_error(s, &failed);
+  if (e == hipSuccess && failed) return DLU_KERNEL_FAILURE;
   return to_status(e);
 }
 
@@ -231,9 +275,8 @@ dlu_status solve_refine(const DeviceFactor &f, int nrhs, const double *d_a, int 
   if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return DLU_RUNTIME_FAILURE;
   SolveCache &c = t_cache[device];
   const size_t nr = static_cast<size_t>(n) * nrhs;
-  void *p_r = nullptr, *p_w = nullptr, *p_r32 = nullptr, *p_nw = nullptr, *p_norms = nullptr;
+  void *p_r = nullptr, *p_r32 = nullptr, *p_nw = nullptr, *p_norms = nullptr;
   hipError_t e = c.get(SolveCache::V64A, nr * sizeof(double), &p_r);
-  if (e == hipSuccess) e = c.get(SolveCache::V64B, nr * sizeof(double), &p_w);
   if (e == hipSuccess) e = c.get(SolveCache::V32A, nr * sizeof(float), &p_r32);
   if (e == hipSuccess) e = c.get(SolveCache::NORMW, 32 * static_cast<size_t>(n) * sizeof(double), &p_nw);
   if (e == hipSuccess) e = c.get(SolveCache::NORMS, (3 * static_cast<size_t>(nrhs) + 1) * sizeof(double), &p_norms);
@@ -254,15 +297,14 @@ dlu_status solve_refine(const DeviceFactor &f, int nrhs, const double *d_a, int 
   while (it < max_iter) {
     ++it;
     if (f.fp32) {
-      // the float work buffer reuses the double one (n*nrhs floats fit in n*nrhs doubles)
       e = launch_to_float(r, n, r32, n, n, nrhs, s);
       if (e == hipSuccess)
         e = getrs_core<float>(n, nrhs, static_cast<const float *>(f.lu), n, f.d_perm,
-                              f.d_perm + n, r32, n, static_cast<float *>(p_w), s);
+                              f.d_perm + n, r32, n, c, s);
       if (e == hipSuccess) e = launch_add_correction<float>(d_x, ldx, r32, n, n, nrhs, s);
     } else {
       e = getrs_core<double>(n, nrhs, static_cast<const double *>(f.lu), n, f.d_perm,
-                             f.d_perm + n, r, n, static_cast<double *>(p_w), s);
+                             f.d_perm + n, r, n, c, s);
       if (e == hipSuccess) e = launch_add_correction<double>(d_x, ldx, r, n, n, nrhs, s);
     }
     if (e == hipSuccess)
@@ -274,6 +316,9 @@ dlu_status solve_refine(const DeviceFactor &f, int nrhs, const double *d_a, int 
       break;
     }
   }
+  bool failed = false;
+  if ((e = c.chain_error(s, &failed)) != hipSuccess) return to_status(e);
+  if (failed) return DLU_KERNEL_FAILURE;
   if (iterations) *iterations = it;
   if (converged) *converged = conv;
   if (bwd)

@@ -24,113 +24,164 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const T *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------------------------
-// X(r0 : r0+w, c) := D^{-1} X(r0 : r0+w, c) for a strip of 16 right-hand sides c, with D the
-// w x w diagonal block of the factor at (r0, r0): unit lower (L) or non-unit upper (U), w <= 128.
-// Every workgroup forms the inverses of D's 16 x 16 diagonal sub-blocks itself (one thread per
-// column, by substitution), then runs block substitution on MFMA: X_b := inv(D_bb) X_b, and the
-// rows still to be solved (below for L, above for U) lose D(rows, b) X_b, with that slice of D
-// staged in LDS.  Rows beyond w are zero-padded and never written back.
+// A whole triangular solve in one launch: X := T^{-1} X with T the unit lower (L) or non-unit
+// upper (U) triangle of an n x n factor and X n x nrhs (nrhs <= 16), rows padded in x to a
+// multiple of 64.  Rows go in blocks of 64, one workgroup each.  Workgroups take blocks in
+// dependency order (L: top first, U: bottom first) from a ticket counter, so every block a
+// workgroup waits on is held by a workgroup that already runs.  A workgroup first inverts the
+// four 16 x 16 diagonal sub-blocks of its block (into LDS), then subtracts each finished block's
+// contribution as soon as that block's flag carries this launch's epoch (MFMA; wave w owns rows
+// 16w..16w+15 of the block; the A operands of the next block are loaded before its wait), then
+// solves its diagonal block by 16-row substitution, writes its rows and raises its flag.  Rows
+// at and beyond n are zero before the diagonal solve and stay zero.  Waits are bounded: after
+// spin_limit ticks of the 100 MHz clock a workgroup sets *err and goes on.
 // ---------------------------------------------------------------------------------------------
-template <typename T, bool UPPER>
-__global__ __launch_bounds__(256) void block_trsv_kernel(const T *__restrict__ lu, int lda,
-                                                        int r0, int w, T *__restrict__ x,
-                                                        int ldx, int nrhs) {
-  typedef Mfma<T> MF;
-  constexpr int NB = 128, NS = 16;
-  __shared__ T X[NB][NS + 2];
-  __shared__ T invD[NB / 16][16][17];
-  __shared__ T Ds[NB][17];
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  const int cs = blockIdx.x * NS;
-  const int ncol = (nrhs - cs) < NS ? (nrhs - cs) : NS;
-  const int nblk = (w + 15) / 16;
-  const int rows = nblk * 16;
-  const T *D = lu + static_cast<size_t>(r0) + static_cast<size_t>(r0) * lda;
-
-  for (int idx = t; idx < rows * NS; idx += 256) {
-    const int i = idx % rows, col = idx / rows;
-    X[i][col] = (i < w && col < ncol)
-                    ? x[static_cast<size_t>(r0 + i) + static_cast<size_t>(cs + col) * ldx]
-                    : T(0);
-  }
-  if (t < 16 * nblk) {
-    const int blk = t >> 4, col = t & 15;
-    const int b0 = blk * 16;
-    const int nr = (w - b0) < 16 ? (w - b0) : 16;
-    T v[16];
-    if (!UPPER) {  // unit lower: forward substitution for column `col` of the inverse
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        T sum = (r == col) ? T(1) : T(0);
-        if (r > col && r < nr) {
-#pragma unroll
-          for (int q = 0; q < 16; ++q)
-            if (q >= col && q < r)
-              sum -= D[static_cast<size_t>(b0 + r) + static_cast<size_t>(b0 + q) * lda] * v[q];
-        }
-        v[r] = (r < nr) ? sum : T(0);
-      }
-    } else {  // non-unit upper: back substitution
-#pragma unroll
-      for (int r = 15; r >= 0; --r) {
-        T val = T(0);
-        if (r < nr && col < nr && r <= col) {
-          T sum = (r == col) ? T(1) : T(0);
-#pragma unroll
-          for (int q = 0; q < 16; ++q)
-            if (q > r && q <= col)
-              sum -= D[static_cast<size_t>(b0 + r) + static_cast<size_t>(b0 + q) * lda] * v[q];
-          val = sum / D[static_cast<size_t>(b0 + r) + static_cast<size_t>(b0 + r) * lda];
-        }
-        v[r] = val;
-      }
+__device__ __forceinline__ void chain_wait(const int *flag, int epoch, long long limit, int *err) {
+  const long long t0 = static_cast<long long>(wall_clock64());
+  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+    if (static_cast<long long>(wall_clock64()) - t0 > limit) {
+      atomicExch(err, 1);
+      return;
     }
-#pragma unroll
-    for (int r = 0; r < 16; ++r) invD[blk][r][col] = v[r];
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void trsv_chain_kernel(
+    const T *__restrict__ lu, int lda, int n, T *__restrict__ x, int ldx, int nrhs,
+    int *__restrict__ flags, unsigned long long *__restrict__ ticket,
+    unsigned long long ticket_base, int epoch, long long spin_limit, int *__restrict__ err) {
+  typedef Mfma<T> MF;
+  typedef typename MF::acc_t acc_t;
+  __shared__ int s_blk;
+  __shared__ T s_d[4][16][17];  // diagonal sub-blocks, then their inverses
+  __shared__ T s_x[16][17];     // the sub-block just solved
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nblk = (n + 63) / 64;
+  if (t == 0) {
+    const unsigned long long tk = atomicAdd(ticket, 1ull) - ticket_base;
+    s_blk = tk < static_cast<unsigned long long>(nblk) ? static_cast<int>(tk) : -1;
+    if (s_blk < 0) atomicExch(err, 2);  // the counter and the host's base disagree
   }
   __syncthreads();
+  if (s_blk < 0) return;
+  const int i = UPPER ? nblk - 1 - s_blk : s_blk;
+  const int r0 = 64 * i;
+  const int col = lr < nrhs ? lr : nrhs - 1;  // columns past nrhs repeat the last, never stored
+  const int last = n - 1;
 
-  const int lr = lane & 15, lk = lane >> 4;
-  for (int step = 0; step < nblk; ++step) {
-    const int blk = UPPER ? nblk - 1 - step : step;
-    const int b0 = blk * 16;
-    const int other = UPPER ? b0 : rows - (b0 + 16);  // rows still to be solved
-    const int obase = UPPER ? 0 : b0 + 16;
-    for (int idx = t; idx < other * 16; idx += 256) {
-      const int rr = idx % other, q = idx / other;
-      const int row = obase + rr, colq = b0 + q;
-      Ds[rr][q] = (row < w && colq < w)
-                      ? D[static_cast<size_t>(row) + static_cast<size_t>(colq) * lda]
-                      : T(0);
+  // 1. Diagonal sub-blocks into LDS (entries outside [0, n) read a clamped address, then become
+  // those of the identity), inverted in place by one thread per sub-block column.
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int idx = 256 * j + t;
+    const int sb = idx >> 8, q = (idx >> 4) & 15, r = idx & 15;
+    const int row = r0 + 16 * sb + r, cl = r0 + 16 * sb + q;
+    const T v = lu[static_cast<size_t>(row < n ? row : last) +
+                   static_cast<size_t>(cl < n ? cl : last) * lda];
+    s_d[sb][r][q] = (row < n && cl < n) ? v : (r == q ? T(1) : T(0));
+  }
+  __syncthreads();
+  T inv[16];
+  if (t < 64) {
+    const int sb = t >> 4, c = t & 15;
+    if (!UPPER) {  // unit lower: forward substitution on e_c
+#pragma unroll
+      for (int r = 0; r < 16; ++r) inv[r] = (r == c) ? T(1) : T(0);
+#pragma unroll
+      for (int q = 0; q < 15; ++q)
+#pragma unroll
+        for (int r = q + 1; r < 16; ++r) inv[r] -= s_d[sb][r][q] * inv[q];
+    } else {  // non-unit upper: back substitution on e_c
+#pragma unroll
+      for (int r = 0; r < 16; ++r) inv[r] = (r == c) ? T(1) : T(0);
+#pragma unroll
+      for (int q = 15; q >= 0; --q) {
+        inv[q] = inv[q] / s_d[sb][q][q];
+#pragma unroll
+        for (int r = 0; r < q; ++r) inv[r] -= s_d[sb][r][q] * inv[q];
+      }
     }
-    if (wave == 0) {
-      typename MF::acc_t d = {T(0), T(0), T(0), T(0)};
+  }
+  __syncthreads();
+  if (t < 64) {
 #pragma unroll
-      for (int k4 = 0; k4 < 16; k4 += 4)
-        d = MF::mma(invD[blk][lr][k4 + lk], X[b0 + k4 + lk][lr], d);
+    for (int r = 0; r < 16; ++r) s_d[t >> 4][r][t & 15] = inv[r];
+  }
+
+  // 2. This wave's 16 rows of the right-hand side, as an MFMA result tile.
+  const int trow = r0 + 16 * wave;  // first row of this wave's tile
+  acc_t acc;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) X[b0 + MF::acc_row(lane, r)][lr] = d[r];
+  for (int r = 0; r < 4; ++r)
+    acc[r] = x[static_cast<size_t>(trow + MF::acc_row(lane, r)) + static_cast<size_t>(col) * ldx];
+
+  // 3. Contributions of the blocks solved before this one.
+  const int arow = trow + lr < n ? trow + lr : last;
+  const T *arow_p = lu + static_cast<size_t>(arow);
+  const int nj = UPPER ? nblk - 1 - i : i;
+  T a_next[16];
+  auto load_a = [&](int j, T (&dst)[16]) {
+#pragma unroll
+    for (int s4 = 0; s4 < 16; ++s4) {
+      const int kc = 64 * j + 4 * s4 + lk;
+      dst[s4] = -arow_p[static_cast<size_t>(kc < n ? kc : last) * lda];
+    }
+  };
+  if (nj > 0) load_a(UPPER ? nblk - 1 : 0, a_next);
+  for (int d = 0; d < nj; ++d) {
+    const int j = UPPER ? nblk - 1 - d : d;
+    T a_cur[16];
+#pragma unroll
+    for (int s4 = 0; s4 < 16; ++s4) a_cur[s4] = a_next[s4];
+    if (d + 1 < nj) load_a(UPPER ? j - 1 : j + 1, a_next);
+    chain_wait(flags + j, epoch, spin_limit, err);
+    const T *xj = x + static_cast<size_t>(64 * j) + static_cast<size_t>(col) * ldx;
+    T b[16];
+#pragma unroll
+    for (int s4 = 0; s4 < 16; ++s4) b[s4] = xj[4 * s4 + lk];
+#pragma unroll
+    for (int s4 = 0; s4 < 16; ++s4) acc = MF::mma(a_cur[s4], b[s4], acc);
+  }
+  // Rows past n: zero (their contributions read clamped entries).
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[r] = (trow + MF::acc_row(lane, r) < n) ? acc[r] : T(0);
+
+  // 4. The diagonal block: sub-block sb := inv(D_sb) acc_sb, then the later sub-blocks lose
+  // D(their rows, sb) X_sb.  An MFMA result tile is the B operand of the next product when k
+  // runs in the result's row order (k-step s of lane l is row acc_row(l, s)).
+  for (int step = 0; step < 4; ++step) {
+    const int sb = UPPER ? 3 - step : step;
+    if (wave == sb) {
+      acc_t dd = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) dd = MF::mma(s_d[sb][lr][MF::acc_row(lane, s4)], acc[s4], dd);
+      acc = dd;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = MF::acc_row(lane, r);
+        s_x[rr][lr] = acc[r];
+        if (lr < nrhs) x[static_cast<size_t>(trow + rr) + static_cast<size_t>(lr) * ldx] = acc[r];
+      }
     }
     __syncthreads();
-    for (int tb = wave; tb < other / 16; tb += 4) {
-      const int rt = obase + 16 * tb;
-      typename MF::acc_t d;
+    if (UPPER ? wave < sb : wave > sb) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) d[r] = X[rt + MF::acc_row(lane, r)][lr];
-#pragma unroll
-      for (int k4 = 0; k4 < 16; k4 += 4)
-        d = MF::mma(-Ds[16 * tb + lr][k4 + lk], X[b0 + k4 + lk][lr], d);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) X[rt + MF::acc_row(lane, r)][lr] = d[r];
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const int kc = r0 + 16 * sb + 4 * s4 + lk;
+        const T a = arow_p[static_cast<size_t>(kc < n ? kc : last) * lda];
+        acc = MF::mma(-a, s_x[4 * s4 + lk][lr], acc);
+      }
     }
     __syncthreads();
   }
 
-  for (int idx = t; idx < w * ncol; idx += 256) {
-    const int i = idx % w, col = idx / w;
-    x[static_cast<size_t>(r0 + i) + static_cast<size_t>(cs + col) * ldx] = X[i][col];
-  }
+  // 5. Publish: the block's rows are out; raise its flag.
+  __threadfence();
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(flags + i, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -252,17 +303,19 @@ hipError_t launch_gather_rows(const T *src, int lds, T *dst, int ldd, const int 
 }
 
 template <typename T>
-hipError_t launch_block_trsv(const T *lu, int lda, int r0, int w, T *x, int ldx, int nrhs,
-                             bool upper, hipStream_t s) {
-  if (w <= 0 || nrhs <= 0) return hipSuccess;
-  if (w > 128) return hipErrorInvalidValue;
-  const dim3 grid((nrhs + 15) / 16), block(256);
+hipError_t launch_trsv_chain(const T *lu, int lda, int n, T *x, int ldx, int nrhs, bool upper,
+                             const ChainState &cs, hipStream_t s) {
+  if (n <= 0 || nrhs <= 0) return hipSuccess;
+  if (nrhs > 16 || ldx < (n + 63) / 64 * 64) return hipErrorInvalidValue;
+  const int nblk = (n + 63) / 64;
   if (upper)
-    hipLaunchKernelGGL((block_trsv_kernel<T, true>), grid, block, 0, s, lu, lda, r0, w, x, ldx,
-                       nrhs);
+    hipLaunchKernelGGL((trsv_chain_kernel<T, true>), dim3(nblk), dim3(256), 0, s, lu, lda, n, x,
+                       ldx, nrhs, cs.flags, cs.ticket, cs.ticket_base, cs.epoch, cs.spin_limit,
+                       cs.err);
   else
-    hipLaunchKernelGGL((block_trsv_kernel<T, false>), grid, block, 0, s, lu, lda, r0, w, x, ldx,
-                       nrhs);
+    hipLaunchKernelGGL((trsv_chain_kernel<T, false>), dim3(nblk), dim3(256), 0, s, lu, lda, n, x,
+                       ldx, nrhs, cs.flags, cs.ticket, cs.ticket_base, cs.epoch, cs.spin_limit,
+                       cs.err);
   return hipGetLastError();
 }
 
@@ -318,9 +371,9 @@ template hipError_t launch_gather_rows<float>(const float *, int, float *, int, 
                                               int, hipStream_t);
 template hipError_t launch_gather_rows<double>(const double *, int, double *, int, const int *,
                                                int, int, hipStream_t);
-template hipError_t launch_block_trsv<float>(const float *, int, int, int, float *, int, int, bool,
-                                             hipStream_t);
-template hipError_t launch_block_trsv<double>(const double *, int, int, int, double *, int, int,
-                                              bool, hipStream_t);
+template hipError_t launch_trsv_chain<float>(const float *, int, int, float *, int, int, bool,
+                                             const ChainState &, hipStream_t);
+template hipError_t launch_trsv_chain<double>(const double *, int, int, double *, int, int, bool,
+                                              const ChainState &, hipStream_t);
 
 }  // namespace dlu

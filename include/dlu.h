@@ -71,7 +71,7 @@ typedef struct dlu_options {
   long long spin_limit_ticks; /* bound on any in-kernel wait, 100 MHz ticks (default 2e8 = 2 s) */
   int profile;     /* nonzero: time every panel / solve / update launch with HIP events */
   int panel_cus;   /* CUs reserved for panel kernels while updates run on the rest:
-                      0 = automatic, > 0 = that many, < 0 = no reservation */
+                      > 0 = that many; 0 (automatic) and < 0 = no reservation */
   int rows_per_lane; /* panel rows held per lane: 0 = automatic, else 1, 2 or 4 */
   int merge_rows;    /* pairs of panels share one trailing GEMM (K = 2 nb) while at least this
                         many rows remain below them: 0 = automatic (3000), < 0 = never */
@@ -92,7 +92,7 @@ typedef struct dlu_report {
   int update_launches;
   double update_seconds;
   double update_flops;
-  int large_update_launches;   /* the subset that ran the 128 x 128-tile kernel */
+  int large_update_launches;   /* the subset over at least 512 blocks of 128 x 128 of C */
   double large_update_seconds;
   double large_update_flops;
   int panel_launches;
@@ -102,6 +102,8 @@ typedef struct dlu_report {
   /* panel kernel, workgroup 0, summed over all steps (profile only): local argmax, publish,
    * candidate exchange, reduction, pivot-row fetch, interchange+scale, rank-1 update, tail */
   double panel_phase_seconds[8];
+  double large_update_bytes;   /* least memory traffic of the large updates: C read and written
+                                  once, A and B read once (profile only) */
 } dlu_report;
 
 /* dlu_getrf_device with options and a report (either may be NULL).  `margins`, when not NULL,
