@@ -141,14 +141,15 @@ template <typename T>
 hipError_t launch_gather_rows(const T *src, int lds, T *dst, int ldd, const int *perm, int n,
                               int nrhs, hipStream_t s);
 
-// Device state of the one-launch triangular solves: per-block completion flags (holding the
-// epoch of the launch that finished the block), a ticket counter that only grows (each launch
-// takes ticket_base .. ticket_base + blocks - 1) and an error word set by an expired wait.
+// Device state of the one-launch triangular solves: the exchange words of solved rows (tagged
+// with the epoch of the launch that wrote them; 16 columns x (1 or 2) words per row, rows of the
+// padded right-hand side), a ticket counter that only grows (each launch takes ticket_base ..
+// ticket_base + blocks - 1) and an error word set by an expired wait.
 struct ChainState {
-  int *flags;
+  unsigned long long *words;
   unsigned long long *ticket;
   unsigned long long ticket_base;
-  int epoch;
+  unsigned epoch;
   long long spin_limit;  // 100 MHz ticks
   int *err;
 };

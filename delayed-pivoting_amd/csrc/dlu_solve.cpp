@@ -21,12 +21,12 @@ constexpr long long kChainSpin = 200000000ll;  // bounded waits: 2 s of the 100 
 
 // Device buffers of the solve side, grown on demand and kept per device.
 struct SolveCache {
-  enum { A32, A64, V64A, V64B, V32A, V32B, NORMW, NORMS, PERM, WORK, CHAIN, COUNT };
+  enum { A32, A64, V64A, V64B, V32A, V32B, NORMW, NORMS, PERM, WORK, CHAIN, WORDS, COUNT };
   void *buf[COUNT] = {};
   size_t cap[COUNT] = {};
-  int epoch = 0;                       // last epoch handed to a triangular-solve launch
+  unsigned epoch = 0;                  // last epoch handed to a triangular-solve launch
   unsigned long long ticket_base = 0;  // tickets taken so far from the CHAIN counter
-  bool chain_fresh = false;            // CHAIN was (re)allocated and holds no zeros yet
+  bool chain_fresh = false;            // CHAIN or WORDS was (re)allocated: not cleared yet
   hipError_t get(int i, size_t bytes, void **out) {
     if (bytes == 0) bytes = 16;
     if (cap[i] < bytes) {
@@ -36,28 +36,31 @@ struct SolveCache {
       hipError_t e = hipMalloc(&buf[i], bytes);
       if (e != hipSuccess) return e;
       cap[i] = bytes;
-      if (i == CHAIN) chain_fresh = true;
+      if (i == CHAIN || i == WORDS) chain_fresh = true;
     }
     *out = buf[i];
     return hipSuccess;
   }
   // Chain state for one triangular-solve launch over `blocks` row blocks, on stream s.  CHAIN
-  // layout: the ticket counter (8 bytes), the error word (4 bytes, then padding), the per-block
-  // flags.  A fresh area is cleared on s itself, so the clearing is ordered before the launch
-  // whatever kind of stream s is.
+  // holds the ticket counter (8 bytes) and the error word; WORDS the tagged exchange words (16
+  // columns x 2 words per padded row).  Fresh areas are cleared on s itself, so the clearing is
+  // ordered before the launch whatever kind of stream s is.
   hipError_t chain(int blocks, ChainState &cs, hipStream_t s) {
-    void *p = nullptr;
-    hipError_t e = get(CHAIN, 16 + sizeof(int) * static_cast<size_t>(blocks), &p);
+    void *p = nullptr, *pw = nullptr;
+    hipError_t e = get(CHAIN, 16, &p);
+    if (e == hipSuccess)
+      e = get(WORDS, sizeof(unsigned long long) * 32 * 64 * static_cast<size_t>(blocks), &pw);
     if (e != hipSuccess) return e;
-    if (chain_fresh || epoch == 0x7FFFFFFF) {  // new area, or flags would repeat an epoch
+    if (chain_fresh || epoch == 0xFFFFFFFFu) {  // new areas, or the tags would repeat an epoch
       if ((e = hipMemsetAsync(p, 0, cap[CHAIN], s)) != hipSuccess) return e;
+      if ((e = hipMemsetAsync(pw, 0, cap[WORDS], s)) != hipSuccess) return e;
       chain_fresh = false;
       epoch = 0;
       ticket_base = 0;
     }
     cs.ticket = static_cast<unsigned long long *>(p);
     cs.err = reinterpret_cast<int *>(static_cast<char *>(p) + 8);
-    cs.flags = reinterpret_cast<int *>(static_cast<char *>(p) + 16);
+    cs.words = static_cast<unsigned long long *>(pw);
     cs.ticket_base = ticket_base;
     cs.epoch = ++epoch;
     cs.spin_limit = kChainSpin;

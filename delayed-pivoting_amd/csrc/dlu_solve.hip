@@ -13,6 +13,12 @@
 namespace dlu {
 namespace {
 
+__device__ __forceinline__ void wave_sync_lds_solve() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void gather_rows_kernel(const T *__restrict__ src, int lds,
                                                          T *__restrict__ dst, int ldd,
@@ -28,35 +34,52 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const T *__restrict__ 
 // upper (U) triangle of an n x n factor and X n x nrhs (nrhs <= 16), rows padded in x to a
 // multiple of 64.  Rows go in blocks of 64, one workgroup each.  Workgroups take blocks in
 // dependency order (L: top first, U: bottom first) from a ticket counter, so every block a
-// workgroup waits on is held by a workgroup that already runs.  A workgroup first inverts the
-// four 16 x 16 diagonal sub-blocks of its block (into LDS), then subtracts each finished block's
-// contribution as soon as that block's flag carries this launch's epoch (MFMA; wave w owns rows
-// 16w..16w+15 of the block; the A operands of the next block are loaded before its wait), then
-// solves its diagonal block by 16-row substitution, writes its rows and raises its flag.  Rows
-// at and beyond n are zero before the diagonal solve and stay zero.  Waits are bounded: after
-// spin_limit ticks of the 100 MHz clock a workgroup sets *err and goes on.
+// workgroup waits on is held by a workgroup that already runs.
+//
+// A solved block travels through xw as tagged exchange words (an LL protocol: 32 payload bits
+// and the launch's 32-bit epoch in each 64-bit word, two words per FP64 value), so a reader
+// spins on the data itself with relaxed loads and no flag, fence or cache invalidation sits on
+// the chain.  Per block: before any wait, wave 0 inverts the four 16 x 16 diagonal sub-blocks
+// (LDS) and loads the off-diagonal sub-blocks of the diagonal block; every wave (16 rows each)
+// subtracts the contribution of each earlier block as its words arrive (MFMA, the A operands of
+// the next block loaded ahead); then wave 0 gathers the 64 rows and solves the diagonal block
+// alone, in registers, and publishes the rows.  Rows at and beyond n are zero before the
+// diagonal solve and stay zero.  Waits are bounded: after spin_limit ticks of the 100 MHz
+// clock a wave sets *err and goes on.
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ void chain_wait(const int *flag, int epoch, long long limit, int *err) {
-  const long long t0 = static_cast<long long>(wall_clock64());
-  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-    if (static_cast<long long>(wall_clock64()) - t0 > limit) {
-      atomicExch(err, 1);
-      return;
-    }
-    __builtin_amdgcn_s_sleep(1);
+template <typename T>
+struct ChainWords;
+template <>
+struct ChainWords<double> {
+  static constexpr int n = 2;
+  static __device__ __forceinline__ unsigned part(double x, int i) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+    return i ? static_cast<unsigned>(b >> 32) : static_cast<unsigned>(b);
   }
-}
+  static __device__ __forceinline__ double join(unsigned lo, unsigned hi) {
+    return __longlong_as_double(
+        static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+  }
+};
+template <>
+struct ChainWords<float> {
+  static constexpr int n = 1;
+  static __device__ __forceinline__ unsigned part(float x, int) { return __float_as_uint(x); }
+  static __device__ __forceinline__ float join(unsigned lo, unsigned) { return __uint_as_float(lo); }
+};
 
 template <typename T, bool UPPER>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(
     const T *__restrict__ lu, int lda, int n, T *__restrict__ x, int ldx, int nrhs,
-    int *__restrict__ flags, unsigned long long *__restrict__ ticket,
-    unsigned long long ticket_base, int epoch, long long spin_limit, int *__restrict__ err) {
+    unsigned long long *__restrict__ xw, unsigned long long *__restrict__ ticket,
+    unsigned long long ticket_base, unsigned epoch, long long spin_limit, int *__restrict__ err) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
+  typedef ChainWords<T> CW;
+  constexpr int WPV = CW::n;
   __shared__ int s_blk;
   __shared__ T s_d[4][16][17];  // diagonal sub-blocks, then their inverses
-  __shared__ T s_x[16][17];     // the sub-block just solved
+  __shared__ T s_acc[64][17];   // the block's right-hand sides after all contributions
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nblk = (n + 63) / 64;
@@ -71,6 +94,7 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
   const int r0 = 64 * i;
   const int col = lr < nrhs ? lr : nrhs - 1;  // columns past nrhs repeat the last, never stored
   const int last = n - 1;
+  const long long t0 = static_cast<long long>(wall_clock64());
 
   // 1. Diagonal sub-blocks into LDS (entries outside [0, n) read a clamped address, then become
   // those of the identity), inverted in place by one thread per sub-block column.
@@ -84,19 +108,17 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
     s_d[sb][r][q] = (row < n && cl < n) ? v : (r == q ? T(1) : T(0));
   }
   __syncthreads();
-  T inv[16];
-  if (t < 64) {
-    const int sb = t >> 4, c = t & 15;
-    if (!UPPER) {  // unit lower: forward substitution on e_c
+  if (wave == 0) {
+    const int sb = lane >> 4, c = lr;
+    T inv[16];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) inv[r] = (r == c) ? T(1) : T(0);
+    for (int r = 0; r < 16; ++r) inv[r] = (r == c) ? T(1) : T(0);
+    if (!UPPER) {  // unit lower: forward substitution on e_c
 #pragma unroll
       for (int q = 0; q < 15; ++q)
 #pragma unroll
         for (int r = q + 1; r < 16; ++r) inv[r] -= s_d[sb][r][q] * inv[q];
     } else {  // non-unit upper: back substitution on e_c
-#pragma unroll
-      for (int r = 0; r < 16; ++r) inv[r] = (r == c) ? T(1) : T(0);
 #pragma unroll
       for (int q = 15; q >= 0; --q) {
         inv[q] = inv[q] / s_d[sb][q][q];
@@ -104,26 +126,46 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
         for (int r = 0; r < q; ++r) inv[r] -= s_d[sb][r][q] * inv[q];
       }
     }
-  }
-  __syncthreads();
-  if (t < 64) {
+    wave_sync_lds_solve();
 #pragma unroll
-    for (int r = 0; r < 16; ++r) s_d[t >> 4][r][t & 15] = inv[r];
+    for (int r = 0; r < 16; ++r) s_d[sb][r][c] = inv[r];
+  }
+  // Wave 0: the six off-diagonal 16 x 16 tiles of the diagonal block as A operands, negated, in
+  // the k order of an MFMA result tile (k-step s of lane l is row acc_row(l, s) of the B tile).
+  // Lower: tile (tr, tc) with tr > tc; upper: tr < tc.  Entries outside [0, n) are never used
+  // with a nonzero B (rows past n are zero), so clamped addresses suffice.
+  T dt[6][4];
+  if (wave == 0) {
+    int k6 = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (UPPER ? a < b : a > b) {
+          const int row = r0 + 16 * a + lr;
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4) {
+            const int cl = r0 + 16 * b + MF::acc_row(lane, s4);
+            dt[k6][s4] = -lu[static_cast<size_t>(row < n ? row : last) +
+                             static_cast<size_t>(cl < n ? cl : last) * lda];
+          }
+          ++k6;
+        }
   }
 
   // 2. This wave's 16 rows of the right-hand side, as an MFMA result tile.
-  const int trow = r0 + 16 * wave;  // first row of this wave's tile
+  const int trow = r0 + 16 * wave;
   acc_t acc;
 #pragma unroll
   for (int r = 0; r < 4; ++r)
     acc[r] = x[static_cast<size_t>(trow + MF::acc_row(lane, r)) + static_cast<size_t>(col) * ldx];
 
-  // 3. Contributions of the blocks solved before this one.
+  // 3. Contributions of the blocks solved before this one, as their words arrive.
   const int arow = trow + lr < n ? trow + lr : last;
   const T *arow_p = lu + static_cast<size_t>(arow);
   const int nj = UPPER ? nblk - 1 - i : i;
   T a_next[16];
-  auto load_a = [&](int j, T (&dst)[16]) {
+  auto load_a = [&](int j, T(&dst)[16]) {
 #pragma unroll
     for (int s4 = 0; s4 < 16; ++s4) {
       const int kc = 64 * j + 4 * s4 + lk;
@@ -131,57 +173,98 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
     }
   };
   if (nj > 0) load_a(UPPER ? nblk - 1 : 0, a_next);
+  bool late = false;
   for (int d = 0; d < nj; ++d) {
     const int j = UPPER ? nblk - 1 - d : d;
     T a_cur[16];
 #pragma unroll
     for (int s4 = 0; s4 < 16; ++s4) a_cur[s4] = a_next[s4];
     if (d + 1 < nj) load_a(UPPER ? j - 1 : j + 1, a_next);
-    chain_wait(flags + j, epoch, spin_limit, err);
-    const T *xj = x + static_cast<size_t>(64 * j) + static_cast<size_t>(col) * ldx;
-    T b[16];
+    // B operand k-step s4: row 4 s4 + lk of block j, column lr.
+    const unsigned long long *wj =
+        xw + (static_cast<size_t>(64 * j + lk) * 16 + lr) * WPV;
+    unsigned lo[16], hi[16];
+    for (;;) {
+      bool fresh = true;
 #pragma unroll
-    for (int s4 = 0; s4 < 16; ++s4) b[s4] = xj[4 * s4 + lk];
-#pragma unroll
-    for (int s4 = 0; s4 < 16; ++s4) acc = MF::mma(a_cur[s4], b[s4], acc);
-  }
-  // Rows past n: zero (their contributions read clamped entries).
-#pragma unroll
-  for (int r = 0; r < 4; ++r) acc[r] = (trow + MF::acc_row(lane, r) < n) ? acc[r] : T(0);
-
-  // 4. The diagonal block: sub-block sb := inv(D_sb) acc_sb, then the later sub-blocks lose
-  // D(their rows, sb) X_sb.  An MFMA result tile is the B operand of the next product when k
-  // runs in the result's row order (k-step s of lane l is row acc_row(l, s)).
-  for (int step = 0; step < 4; ++step) {
-    const int sb = UPPER ? 3 - step : step;
-    if (wave == sb) {
-      acc_t dd = {T(0), T(0), T(0), T(0)};
-#pragma unroll
-      for (int s4 = 0; s4 < 4; ++s4) dd = MF::mma(s_d[sb][lr][MF::acc_row(lane, s4)], acc[s4], dd);
-      acc = dd;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int rr = MF::acc_row(lane, r);
-        s_x[rr][lr] = acc[r];
-        if (lr < nrhs) x[static_cast<size_t>(trow + rr) + static_cast<size_t>(lr) * ldx] = acc[r];
+      for (int s4 = 0; s4 < 16; ++s4) {
+        const unsigned long long w0 = __hip_atomic_load(wj + 4 * 16 * WPV * s4, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+        lo[s4] = static_cast<unsigned>(w0);
+        fresh = fresh && static_cast<unsigned>(w0 >> 32) == epoch;
+        if (WPV == 2) {
+          const unsigned long long w1 = __hip_atomic_load(
+              wj + 4 * 16 * WPV * s4 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          hi[s4] = static_cast<unsigned>(w1);
+          fresh = fresh && static_cast<unsigned>(w1 >> 32) == epoch;
+        } else {
+          hi[s4] = 0u;
+        }
       }
-    }
-    __syncthreads();
-    if (UPPER ? wave < sb : wave > sb) {
-#pragma unroll
-      for (int s4 = 0; s4 < 4; ++s4) {
-        const int kc = r0 + 16 * sb + 4 * s4 + lk;
-        const T a = arow_p[static_cast<size_t>(kc < n ? kc : last) * lda];
-        acc = MF::mma(-a, s_x[4 * s4 + lk][lr], acc);
+      if (__all(fresh)) break;
+      if (late || static_cast<long long>(wall_clock64()) - t0 > spin_limit) {
+        if (!late && lane == 0) atomicExch(err, 1);
+        late = true;
+        break;
       }
+      __builtin_amdgcn_s_sleep(1);
     }
-    __syncthreads();
+#pragma unroll
+    for (int s4 = 0; s4 < 16; ++s4) acc = MF::mma(a_cur[s4], CW::join(lo[s4], hi[s4]), acc);
   }
 
-  // 5. Publish: the block's rows are out; raise its flag.
-  __threadfence();
+  // 4. The 64 rows to wave 0 (rows past n zeroed), which solves the diagonal block alone.
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int rr = 16 * wave + MF::acc_row(lane, r);
+    s_acc[rr][lr] = (r0 + rr < n) ? acc[r] : T(0);
+  }
   __syncthreads();
-  if (t == 0) __hip_atomic_store(flags + i, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (wave != 0) return;
+  acc_t xr[4];
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) xr[tt][r] = s_acc[16 * tt + MF::acc_row(lane, r)][lr];
+  // Substitution over the four sub-blocks: X_b := inv(D_bb) X_b, then X_t -= D_tb X_b.  With
+  // k in the result's row order, a result tile is the B operand of the next product as it is.
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const int b = UPPER ? 3 - st : st;
+    acc_t dd = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) dd = MF::mma(s_d[b][lr][MF::acc_row(lane, s4)], xr[b][s4], dd);
+    xr[b] = dd;
+    int k6 = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (UPPER ? a < c : a > c) {
+          if (c == b) {
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) xr[a] = MF::mma(dt[k6][s4], xr[b][s4], xr[a]);
+          }
+          ++k6;
+        }
+  }
+
+  // 5. Publish: tagged words for the blocks that wait on this one, plain values for the next
+  // kernel.
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = r0 + 16 * tt + MF::acc_row(lane, r);
+      const T v = xr[tt][r];
+      unsigned long long *w = xw + (static_cast<size_t>(row) * 16 + lr) * WPV;
+#pragma unroll
+      for (int p = 0; p < WPV; ++p)
+        __hip_atomic_store(w + p,
+                           (static_cast<unsigned long long>(epoch) << 32) | CW::part(v, p),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lr < nrhs) x[static_cast<size_t>(row) + static_cast<size_t>(lr) * ldx] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -310,11 +393,11 @@ hipError_t launch_trsv_chain(const T *lu, int lda, int n, T *x, int ldx, int nrh
   const int nblk = (n + 63) / 64;
   if (upper)
     hipLaunchKernelGGL((trsv_chain_kernel<T, true>), dim3(nblk), dim3(256), 0, s, lu, lda, n, x,
-                       ldx, nrhs, cs.flags, cs.ticket, cs.ticket_base, cs.epoch, cs.spin_limit,
+                       ldx, nrhs, cs.words, cs.ticket, cs.ticket_base, cs.epoch, cs.spin_limit,
                        cs.err);
   else
     hipLaunchKernelGGL((trsv_chain_kernel<T, false>), dim3(nblk), dim3(256), 0, s, lu, lda, n, x,
-                       ldx, nrhs, cs.flags, cs.ticket, cs.ticket_base, cs.epoch, cs.spin_limit,
+                       ldx, nrhs, cs.words, cs.ticket, cs.ticket_base, cs.epoch, cs.spin_limit,
                        cs.err);
   return hipGetLastError();
 }
