@@ -131,6 +131,35 @@ def bench_mixed(dlu, torch, dev, n, nrhs):
     return out
 
 
+def bench_sweep(dlu, torch, dev, sizes):
+    """Plain FP64 delayed LU over the size sweep of the headline configuration: best of two
+    factorizations after one warm-up, N(0,1) seeded per n, device interface."""
+    out = []
+    for n in sizes:
+        log(f"sweep n={n}")
+        g = torch.Generator(device=dev)
+        g.manual_seed(n)
+        orig = torch.randn((n, n), dtype=torch.float64, device=dev, generator=g)
+        work = torch.empty_like(orig)
+        best = None
+        info = None
+        for rep in range(3):
+            work.copy_(orig)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st, _, _, info, _, _ = dlu.getrf_device(n, n, work.data_ptr(), n)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            if rep > 0 and int(st) == 0:
+                best = t if best is None else min(best, t)
+        out.append({"n": n, "ms": round(best * 1e3, 3) if best else None,
+                    "tflops": round(2.0 * n ** 3 / 3.0 / best / 1e12, 3) if best else None,
+                    "info": info})
+        del orig, work
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_serving(dlu, torch, dev, count, seed=20000):
     """Poisson arrivals of factor + solve requests, n log-uniform in [64, 4096], 1-8 RHS,
     30% reusing an earlier request's resident factor; FP64 and mixed-precision services."""
@@ -356,8 +385,9 @@ def main():
 
     del orig, work
     torch.cuda.empty_cache()
-    c3 = c4 = c5 = None
+    c3 = c4 = c5 = sweep = None
     if not args.no_extra:
+        sweep = bench_sweep(dlu, torch, dev, [2048, 4096, 8192, 16384, 32768])
         c3 = bench_delay_stress(dlu, torch, dev, 8192)
         c4 = bench_mixed(dlu, torch, dev, args.mixed_n, 16)
         c5 = bench_serving(dlu, torch, dev, args.serve_requests)
@@ -375,7 +405,7 @@ def main():
         "roofline": roofline, "phases": phases, "cpu_baseline": cpu, "parity": parity,
         "pivot_report": {k: first[2][k] for k in ("fallbacks", "near_ties", "exact_ties",
                                                   "min_margin", "panels")} if first[2] else None,
-        "c3_delay_stress": c3, "c4_mixed_precision": c4, "c5_serving": c5,
+        "c2_sweep": sweep, "c3_delay_stress": c3, "c4_mixed_precision": c4, "c5_serving": c5,
     }
     print(json.dumps(out), flush=True)
     return 0
