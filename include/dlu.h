@@ -75,9 +75,10 @@ typedef struct dlu_options {
   int rows_per_lane; /* panel rows held per lane: 0 = automatic, else 1, 2 or 4 */
   int merge_rows;    /* pairs of panels share one trailing GEMM (K = 2 nb) while at least this
                         many rows remain below them: 0 = automatic (3000), < 0 = never */
-  int split_update;  /* the far trailing GEMM's last columns run on the panel stream's CUs once
-                        the next panel is done, sized so both streams finish together:
-                        0 = automatic, < 0 = never, > 0 = always that many columns */
+  int split_update;  /* the far trailing GEMM's last columns run on the panel stream once the
+                        next panel is done, sized so both streams finish together: 0 = automatic
+                        (only when panel_cus reserves CUs), < 0 = never, > 0 = always that many
+                        columns */
 } dlu_options;
 
 /* What happened during one factorization. */
