@@ -53,6 +53,9 @@ struct Workspace {
   int cus = 0;
   Control *d_ctl = nullptr;
   Control *h_ctl = nullptr;
+  PanelDone *h_done = nullptr;   // pinned, coherent: written by panel kernels
+  PanelDone *d_done = nullptr;   // the same memory as the device sees it
+  unsigned long long done_seq = 0;
   int *d_ipiv = nullptr;
   int *d_jpiv = nullptr;
   double *d_margins = nullptr;
@@ -100,6 +103,13 @@ hipError_t prepare(Workspace &w, int device, int steps) {
     w.cus = prop.multiProcessorCount;
     if ((e = hipMalloc(&w.d_ctl, sizeof(Control))) != hipSuccess) return e;
     if ((e = hipHostMalloc(&w.h_ctl, sizeof(Control))) != hipSuccess) return e;
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&w.h_done), sizeof(PanelDone),
+                           hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess)
+      return e;
+    std::memset(static_cast<void *>(w.h_done), 0, sizeof(PanelDone));
+    if ((e = hipHostGetDevicePointer(reinterpret_cast<void **>(&w.d_done), w.h_done, 0)) !=
+        hipSuccess)
+      return e;
     if ((e = hipMalloc(&w.d_slots, slot_bytes())) != hipSuccess) return e;
     if ((e = hipMemset(w.d_slots, 0, slot_bytes())) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_rowk, rowk_bytes())) != hipSuccess) return e;
@@ -270,15 +280,40 @@ struct Run {
     p.swap_list = w.d_lists + parity * kListStride;
     p.spin_limit = spin;
     p.phase_ticks = profile ? w.d_phase : nullptr;
+    p.done = w.d_done;
+    p.done_seq = ++w.done_seq;
+    expect_done = p.done_seq;
     hipError_t e = profile ? mark(t_panel, ps, 0.0) : hipSuccess;
     if (e == hipSuccess) e = launch_panel(p, nb, rpl, ps);
     if (e == hipSuccess && profile) e = close(t_panel, ps);
     if (e != hipSuccess) return e;
     ++panels;
-    if ((e = hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost, ps)) !=
-        hipSuccess)
-      return e;
     return hipEventRecord(w.ev_ctl, ps);
+  }
+
+  // The last panel's control block: spin on the sequence number the panel kernel stores into
+  // host memory as its last act.  Should the panel have finished without publishing (its event
+  // completes first), the device copy is read instead.
+  unsigned long long expect_done = 0;
+  hipError_t wait_panel(Control &c) {
+    for (unsigned it = 1;; ++it) {
+      if (__atomic_load_n(&w.h_done->seq, __ATOMIC_ACQUIRE) == expect_done) {
+        c = w.h_done->c;
+        return hipSuccess;
+      }
+      if ((it & 255) == 0) {
+        const hipError_t q = hipEventQuery(w.ev_ctl);
+        if (q == hipSuccess) {
+          if (__atomic_load_n(&w.h_done->seq, __ATOMIC_ACQUIRE) == expect_done) {
+            c = w.h_done->c;
+            return hipSuccess;
+          }
+          return hipMemcpy(&c, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost);
+        }
+        if (q != hipErrorNotReady) return q;
+      }
+      __builtin_ia32_pause();
+    }
   }
 
   // Interchange + U block row + trailing update of columns [c0, c1) for the panel rows
@@ -446,8 +481,8 @@ struct Run {
     int pe = std::min(pb + nb, steps);
     if ((e = panel(pb, pe, 0, parity)) != hipSuccess) return e;
     for (;;) {
-      if ((e = hipEventSynchronize(w.ev_ctl)) != hipSuccess) return e;
-      const Control c = *w.h_ctl;
+      Control c;
+      if ((e = wait_panel(c)) != hipSuccess) return e;
       if (c.error != 0) return hipErrorLaunchFailure;
       const int stop = c.stop;
       const int wdt = stop - pb;
@@ -614,15 +649,20 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   run.split = !(opt && opt->split_update < 0);
   if (opt && opt->split_update > 0) run.forced_split = opt->split_update;
   {
-    // CUs reserved for the panel stream: none by default (measured: the trailing update gains
-    // more from the whole device than the panels lose by waiting for update workgroups to drain
-    // from a CU); opt->panel_cus > 0 reserves that many, which also lets the far GEMM's tail
-    // run on them behind each panel.
+    // CUs reserved for the panel stream.  FP64: none by default (measured: the trailing update
+    // gains more from the whole device than the panels lose by waiting for update workgroups to
+    // drain from a CU).  FP32, whose trailing update is twice as fast, is panel-bound: enough
+    // CUs for one resident workgroup per panel row block (at most half the device), which also
+    // lets the far GEMM's tail run on them behind each panel.  opt->panel_cus > 0 forces a
+    // size, < 0 disables.
     int groups = 0, rpl = 0;
     run.max_groups = std::min(kMaxPanelGroups, w.cus);
     run.panel_shape(0, groups, rpl);
     int want = 0;
+    if (sizeof(T) == 4) want = std::max(16, std::min(((groups + 15) / 16) * 16, w.cus / 2));
     if (opt && opt->panel_cus > 0) want = std::min(opt->panel_cus, w.cus);
+    if (opt && opt->panel_cus < 0) want = 0;
+    if (groups > want && want > 0) want = 0;  // too tall for a partition: share the device
     if (groups > want && want > 0) want = 0;  // too tall for a partition: share the device
     Workspace::Streams *st = nullptr;
     if ((e = stream_pair(w, want, st)) != hipSuccess) {
@@ -668,8 +708,11 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
     e = hipMemcpyAsync(jpiv, w.d_jpiv, steps * sizeof(int), hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess && margins)
     e = hipMemcpyAsync(margins, w.d_margins, steps * sizeof(double), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost, stream);
   e2 = hipStreamSynchronize(stream);
   if (e == hipSuccess) e = e2;
+  if (e == hipSuccess && w.h_ctl->error != 0) e = hipErrorLaunchFailure;  // a wait expired late
   if (e != hipSuccess) {
     *info = -1;
     run.release();

@@ -32,6 +32,14 @@ struct Control {
   unsigned abort_flag;  // set by a panel workgroup whose wait expired; the others leave too
 };
 
+// A panel's control block as the host sees it: workgroup 0 of the panel kernel copies the
+// control block here (pinned, coherent host memory) as its last act, then stores `seq` with
+// system-scope release; the host spins on `seq` instead of waiting for a copy and an event.
+struct PanelDone {
+  Control c;
+  unsigned long long seq;
+};
+
 // Exchange buffer geometry of the panel kernel (in 64-bit words).  Every word carries the step
 // tag of the step that wrote it, so a reader can tell a fresh word from a stale one without any
 // cache-maintenance fence.  A slot starts with the two header words (48-bit payload, 16-bit tag
@@ -93,6 +101,8 @@ struct PanelParams {
   int *swap_list;      // swap_list_ints(nb) ints for this panel (parity selected by the host)
   long long spin_limit;  // bounded wait, in s_memrealtime ticks (100 MHz)
   unsigned long long *phase_ticks;  // optional [8] per-phase tick totals (workgroup 0)
+  PanelDone *done;                  // host-visible copy of the control block (may be null)
+  unsigned long long done_seq;      // value of done->seq once this launch has published
 };
 
 // Launchers (return hipSuccess or the launch error).  T is float or double; both are

@@ -256,6 +256,16 @@ __device__ __forceinline__ void as_word(float &x, int i) { x = __int_as_float(i)
 __device__ __forceinline__ int from_word(double x) { return static_cast<int>(lo32(x)); }
 __device__ __forceinline__ int from_word(float x) { return __float_as_int(x); }
 
+// The panel's last word to the host: the control block, copied into pinned host memory, then
+// the sequence number with system-scope release (the host spins on it).
+__device__ __forceinline__ void publish_done(const Control *ctl, PanelDone *done,
+                                             unsigned long long seq) {
+  if (done == nullptr) return;
+  done->c = *ctl;
+  __threadfence_system();
+  __hip_atomic_store(&done->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -320,6 +330,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
       P.ctl->stop = P.pb;
       P.ctl->zero_column = 0;
       P.swap_list[4 * NB] = 0;
+      publish_done(P.ctl, P.done, P.done_seq);
     }
     return;
   }
@@ -712,6 +723,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
           P.ctl->near_ties += near_ties;
           P.ctl->exact_ties += exact_ties;
           if (min_margin < P.ctl->min_margin) P.ctl->min_margin = min_margin;
+          publish_done(P.ctl, P.done, P.done_seq);
         }
       }
     }
@@ -720,7 +732,10 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
 #undef DLU_PHASE
 
 leave:
-  if (g == 0 && threadIdx.x == 0) P.ctl->stop = P.pb;
+  if (g == 0 && threadIdx.x == 0) {
+    P.ctl->stop = P.pb;
+    publish_done(P.ctl, P.done, P.done_seq);
+  }
 }
 
 template <typename T, int NB>

@@ -70,8 +70,9 @@ typedef struct dlu_options {
   int lookahead;   /* 1 (default): factor panel j+1 while panel j's far update runs; -1: off */
   long long spin_limit_ticks; /* bound on any in-kernel wait, 100 MHz ticks (default 2e8 = 2 s) */
   int profile;     /* nonzero: time every panel / solve / update launch with HIP events */
-  int panel_cus;   /* CUs reserved for panel kernels while updates run on the rest:
-                      > 0 = that many; 0 (automatic) and < 0 = no reservation */
+  int panel_cus;   /* CUs reserved for panel kernels while updates run on the rest: > 0 = that
+                      many, < 0 = none, 0 = automatic (none for FP64; for FP32 one CU per panel
+                      workgroup, at most half the device) */
   int rows_per_lane; /* panel rows held per lane: 0 = automatic, else 1, 2 or 4 */
   int merge_rows;    /* pairs of panels share one trailing GEMM (K = 2 nb) while at least this
                         many rows remain below them: 0 = automatic (3000), < 0 = never */
