@@ -15,9 +15,22 @@ namespace {
 //
 // Workgroup tile BM x BN, 256 threads = 4 waves in a 2 x 2 grid, each wave owns a
 // (BM/2) x (BN/2) block made of 16 x 16 MFMA tiles (Mfma<T>).  K advances in chunks of BK staged
-// through double-buffered LDS; the next chunk's global loads are in flight while the current
-// chunk feeds the MFMA pipe.  Global loads are lane-contiguous (consecutive lanes read
-// consecutive rows of A, consecutive k of B), 16 bytes per lane when the operands allow it.
+// through double-buffered LDS with one barrier per chunk:
+//   k-step 0         the next chunk's global loads are issued;
+//   k-steps 0..KS-2  each step reads the fragments of the following step from LDS, then
+//                    issues its own MFMAs;
+//   k-step KS-1      the next chunk is stored to the other LDS buffer, the barrier is passed,
+//                    the next chunk's first fragments are read, and only then are this step's
+//                    MFMAs issued, so they cover the LDS latency after the barrier.
+// The last chunk is peeled, which keeps every k-step free of branches.
+//
+// The MFMA operands are swapped (the B fragment is the first operand): a lane's accumulator
+// entries then lie in one row of C and 16 consecutive lanes hold 16 consecutive rows, so the
+// column-major C is loaded and stored in whole 16-row runs (one 128-byte line per instruction
+// and 16 lanes in FP64) instead of 16 scattered columns.
+//
+// Global loads are lane-contiguous (consecutive lanes read consecutive rows of A, consecutive k
+// of B), 16 bytes per lane when the operands allow it.
 // ---------------------------------------------------------------------------------------------
 template <typename T, int BM, int BN, int BK, bool VEC, int OCC>
 __global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, int ldc,
@@ -36,6 +49,7 @@ __global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, i
   constexpr int B_LD = BN * BK / (256 * VW);
   constexpr int KS = BK / 4;                  // MFMA k-steps per chunk
   static_assert(A_LD >= 1 && B_LD >= 1, "tile too small for the load width");
+  static_assert(KS % 2 == 0, "fragment slots alternate: an even number of k-steps per chunk");
   __shared__ T As[2][BK][AS];
   __shared__ T Bs[2][BN][BS];
 
@@ -60,10 +74,9 @@ __global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, i
 
   // Global loads are branch-free: an out-of-range element is read from a clamped in-range
   // address, and replaced by zero only when it is stashed in LDS, after the current chunk's
-  // MFMAs, so the loads of the next chunk stay in flight while this one computes.  Row / column
-  // clamps are fixed per thread; the k clamp is needed only in a last, partial chunk.  On the
-  // vector path M and K are multiples of the vector width, so a vector is either wholly inside
-  // or wholly outside.
+  // MFMAs were issued, so the loads stay in flight meanwhile.  Row / column clamps are fixed per
+  // thread; the k clamp is needed only in a last, partial chunk.  On the vector path M and K are
+  // multiples of the vector width, so a vector is either wholly inside or wholly outside.
   typedef T dv_t __attribute__((ext_vector_type(VW)));
   constexpr int AR = BM / VW, BR = BK / VW;  // threads per A column / per B column
   static_assert(256 % AR == 0 && 256 % BR == 0, "load layout");
@@ -109,22 +122,27 @@ __global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, i
     }
   };
 
+  // Accumulator (i, j) entry r of this lane is C(crow(i), ccol(j, r)).
+  const int crow_base = row0 + wm * WM + (lane & 15);
+  const int ccol_base = col0 + wn * WN;
+  auto ccol = [&](int j, int r) { return ccol_base + 16 * j + MF::acc_row(lane, r); };
+
   // The accumulators start from C (loaded together with the first chunk, so its latency is not
   // paid again at the end) and the MFMAs add (-A) * B: the epilogue only stores.
   typename MF::acc_t acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+  for (int i = 0; i < FM; ++i) {
+    const int row = crow_base + 16 * i;
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int col = col0 + wn * WN + 16 * j + (lane & 15);
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = row0 + wm * WM + 16 * i + MF::acc_row(lane, r);
+        const int col = ccol(j, r);
         const bool ok = row < M && col < N;
         const T v = C[static_cast<size_t>(ok ? row : 0) + static_cast<size_t>(ok ? col : 0) * ldc];
         acc[i][j][r] = ok ? v : T(0);
       }
-    }
+  }
 
   T af[2][FM], bf[2][FN];
   auto frags = [&](int buf, int kk, int slot) {
@@ -135,42 +153,58 @@ __global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, i
     for (int j = 0; j < FN; ++j)
       bf[slot][j] = Bs[buf][wn * WN + 16 * j + (lane & 15)][kk + (lane >> 4)];
   };
+  auto mmas = [&](int slot) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(bf[slot][j], af[slot][i], acc[i][j]);
+  };
 
   const int nk = (K + BK - 1) / BK;
   load(0);
   stash(0, 0);
   __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
+  frags(0, 0, 0);
+  for (int kt = 0; kt + 1 < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
-    frags(cur, 0, 0);
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-      // LDS reads for the next k-step are issued before this step's MFMAs.
-      if (s + 1 < KS) frags(cur, 4 * (s + 1), (s + 1) & 1);
       __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[s & 1][i], bf[s & 1][j], acc[i][j]);
+      if (s == 0) load((kt + 1) * BK);
+      if (s + 1 < KS) {
+        frags(cur, 4 * (s + 1), (s + 1) & 1);
+      } else {
+        stash(cur ^ 1, (kt + 1) * BK);
+        __syncthreads();
+        frags(cur ^ 1, 0, 0);
+      }
+      mmas(s & 1);
       __builtin_amdgcn_sched_barrier(0);
     }
-    if (kt + 1 < nk) stash(cur ^ 1, (kt + 1) * BK);
-    __syncthreads();
+  }
+  {
+    const int cur = (nk - 1) & 1;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      __builtin_amdgcn_sched_barrier(0);
+      if (s + 1 < KS) frags(cur, 4 * (s + 1), (s + 1) & 1);
+      mmas(s & 1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
 
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+  for (int i = 0; i < FM; ++i) {
+    const int row = crow_base + 16 * i;
+    if (row >= M) continue;
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int col = col0 + wn * WN + 16 * j + (lane & 15);
-      if (col >= N) continue;
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = row0 + wm * WM + 16 * i + MF::acc_row(lane, r);
-        if (row < M) C[static_cast<size_t>(row) + static_cast<size_t>(col) * ldc] = acc[i][j][r];
+        const int col = ccol(j, r);
+        if (col < N) C[static_cast<size_t>(row) + static_cast<size_t>(col) * ldc] = acc[i][j][r];
       }
-    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
