@@ -399,10 +399,10 @@ template <typename T>
 hipError_t launch_gemm_sub(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
                            int K, hipStream_t s) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
-  // FP64: 64 x 64 tiles at four workgroups per CU outrun 128 x 128 tiles at two for every shape
-  // of this factorization (more waves in flight hide the C, A and B traffic of short K).  FP32
-  // MFMAs retire twice as fast, so large FP32 problems keep the 128 x 128 tile's data reuse.
-  if (sizeof(T) == 4 && gemm_uses_large_tiles(M, N))
+  // Launches of at least 512 tiles of 128 x 128 (the far updates while many rows remain) use
+  // 128 x 128 tiles at two workgroups per CU for their data reuse; smaller launches (the next
+  // panel's block, the last panels) keep 64 x 64 tiles at four per CU, which fill the device.
+  if (gemm_uses_large_tiles(M, N))
     return launch_gemm_tile<T, 128, 128, 16, 2>(c, ldc, a, lda, b, ldb, M, N, K, s);
   return launch_gemm_tile<T, 64, 64, 16, 4>(c, ldc, a, lda, b, ldb, M, N, K, s);
 }

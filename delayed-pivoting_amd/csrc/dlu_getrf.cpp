@@ -469,6 +469,16 @@ struct Run {
     return e;
   }
 
+  // The next panel, queued on `ps` behind everything `us` holds so far (the next panel's block),
+  // as soon as that block's update is queued: the far work of the step is queued after it, so
+  // the host's launch calls for the far work are off the panel chain.
+  hipError_t launch_next(int pb_next, int pe_next, int positioned, int parity) {
+    hipError_t e = hipEventRecord(w.ev_next, us);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ps, w.ev_next, 0);
+    if (e == hipSuccess) e = panel(pb_next, pe_next, positioned, (parity + 1) % kLists);
+    return e;
+  }
+
   // Runs on the panel stream `ps` and the update stream `us`, forked from and joined back to the
   // caller's stream.  Per panel the host reads the control block once to learn where the panel
   // stopped; every pivot decision inside the panel is taken on the device.
@@ -509,7 +519,7 @@ struct Run {
           if ((e = wait_tail(us)) != hipSuccess) return e;
           if ((e = launch_row_interchange(a, lda, b0, pb, list, nb, us)) != hipSuccess) return e;
           if ((e = pair_update(b0, pair.l0, list, pe, pe2, us)) != hipSuccess) return e;
-          if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
+          if ((e = launch_next(pe, pe2, 0, parity)) != hipSuccess) return e;
           const int half = std::max(nb, (n - pe2) / 2 / nb * nb);
           const int cm = std::min(n, pe2 + half);
           if ((e = pair_update(b0, pair.l0, list, pe2, cm, us)) != hipSuccess) return e;
@@ -518,7 +528,7 @@ struct Run {
           // First panel of a pair: only the next panel's columns now (and the previous pair's
           // right half, which the next panel's block does not touch).
           if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
-          if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
+          if ((e = launch_next(pe, pe2, 0, parity)) != hipSuccess) return e;
           if ((e = queue_rest(pe2 - pe)) != hipSuccess) return e;
           pair.open = true;
           pair.b0 = pb;
@@ -528,7 +538,7 @@ struct Run {
           // The next panel's columns first, then the far columns while the next panel runs
           // (their last columns on the panel stream behind it when the GEMM outlasts it).
           if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
-          if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
+          if ((e = launch_next(pe, pe2, 0, parity)) != hipSuccess) return e;
           if ((e = queue_rest(0)) != hipSuccess) return e;
           if ((e = wait_tail(us)) != hipSuccess) return e;
           if ((e = solve(pb, wdt, pe2, n, list, us)) != hipSuccess) return e;
@@ -549,7 +559,7 @@ struct Run {
             *info = steps;
             break;
           }
-          if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
+          if ((e = launch_next(pe, pe2, 0, parity)) != hipSuccess) return e;
         }
         pb = pe;
         pe = pe2;
@@ -568,13 +578,11 @@ struct Run {
         if ((e = launch_fallback_apply(a, lda, m, n, stop, w.d_ipiv, w.d_jpiv, w.d_ctl, us)) !=
             hipSuccess)
           return e;
-        if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
         pb = stop;
         pe = std::min(pb + nb, steps);
+        if ((e = launch_next(pb, pe, 1, parity)) != hipSuccess) return e;
       }
       parity = (parity + 1) % kLists;
-      if ((e = hipStreamWaitEvent(ps, w.ev_next, 0)) != hipSuccess) return e;
-      if ((e = panel(pb, pe, c.zero_column ? 1 : 0, parity)) != hipSuccess) return e;
       if ((e = launch_tail()) != hipSuccess) return e;
     }
     return hipSuccess;
@@ -662,7 +670,6 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
     if (sizeof(T) == 4) want = std::max(16, std::min(((groups + 15) / 16) * 16, w.cus / 2));
     if (opt && opt->panel_cus > 0) want = std::min(opt->panel_cus, w.cus);
     if (opt && opt->panel_cus < 0) want = 0;
-    if (groups > want && want > 0) want = 0;  // too tall for a partition: share the device
     if (groups > want && want > 0) want = 0;  // too tall for a partition: share the device
     Workspace::Streams *st = nullptr;
     if ((e = stream_pair(w, want, st)) != hipSuccess) {

@@ -117,7 +117,7 @@ hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const 
                             int nb, hipStream_t s);
 
 // C(M x N) -= A(M x K) * B(K x N), column-major with leading dimensions ldc, lda, ldb.
-// Large FP32 problems use 128 x 128 tiles, everything else 64 x 64 (gemm_uses_large_tiles tells
+// Large problems use 128 x 128 tiles, everything else 64 x 64 (gemm_uses_large_tiles tells
 // which problems count as large).
 inline bool gemm_uses_large_tiles(int M, int N) {
   return static_cast<long long>((M + 127) / 128) * ((N + 127) / 128) >= 512;
