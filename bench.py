@@ -274,6 +274,9 @@ def main():
     ap.add_argument("--mixed-n", type=int, default=16384)
     ap.add_argument("--serve-requests", type=int, default=1500,
                     help="requests of the serving stream measured per precision mode")
+    ap.add_argument("--gemm", type=int, nargs=2, metavar=("M", "K"), default=None,
+                    help="only time the trailing-update kernel C(M x M) -= A(M x K) B(K x M) "
+                         "(the library's dlu_gemm_sub_device), for counter runs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device to device three times (the reference kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -296,6 +299,25 @@ def main():
     dlu = _load("dlu_amd", os.path.join(PKG, "dlu.py"))
     dlu.lib()
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.gemm:
+        # C, A and B column-major with leading dimension M (rows of each tensor are columns).
+        m, kk = args.gemm
+        c = torch.randn((m, m), dtype=torch.float64, device="cuda")
+        a = torch.randn((kk, m), dtype=torch.float64, device="cuda")
+        b = torch.randn((m, m), dtype=torch.float64, device="cuda")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        best = float("inf")
+        for _ in range(max(1, args.steps)):
+            e0.record()
+            st = dlu.gemm_sub_device(m, m, kk, c.data_ptr(), a.data_ptr(), b.data_ptr(), m)
+            e1.record()
+            torch.cuda.synchronize()
+            if st != dlu.Status.SUCCESS:
+                raise RuntimeError(dlu.status_string(st))
+            best = min(best, e0.elapsed_time(e1) / 1e3)
+        print(json.dumps({"gemm": {"M": m, "N": m, "K": kk, "best_ms": round(best * 1e3, 4),
+                                   "tflops": round(2.0 * m * m * kk / best / 1e12, 2)}}), flush=True)
+        return 0
     dev = torch.device("cuda", 0)
     n, nb = args.n, args.nb
     flops = 2.0 * n ** 3 / 3.0
