@@ -94,12 +94,22 @@ def bench_delay_stress(dlu, torch, dev, n):
 
     t, (st, ipiv, jpiv, info, rep, _) = _timed(torch, run)
     delayed = np.sort(dlu.net_order(jpiv, info, n)[info:])
+    # One more, profiled run: the full-pivot searches' own time and the bytes they must read.
+    work.copy_(orig)
+    _, _, _, _, prep, _ = dlu.getrf_device(n, n, work.data_ptr(), n, options={"nb": 0, "profile": 1})
+    torch.cuda.synchronize()
+    search = {"launches": prep["search_launches"], "ms": round(prep["search_seconds"] * 1e3, 2),
+              "bytes": prep["search_bytes"],
+              "gb_per_s": round(prep["search_bytes"] / max(prep["search_seconds"], 1e-12) / 1e9, 1),
+              "materialize_and_update_ms": round(prep["update_seconds"] * 1e3, 2),
+              "panel_ms": round(prep["panel_seconds"] * 1e3, 2)}
     return {"workload": f"delay-stress FP64 n={n} (5% zero cols, 5% x1e-14, rank-32 64x64 block)",
             "ms": round(t * 1e3, 2), "tflops_2n3_3": round(2.0 * n ** 3 / 3 / t / 1e12, 3),
             "info": info, "expected_info": n - len(zero_cols),
             "delayed_set_equals_zero_columns": bool(info == n - len(zero_cols) and
                                                     np.array_equal(delayed, zero_cols)),
             "fallbacks": rep["fallbacks"], "panels": rep["panels"], "status": int(st),
+            "full_pivot_search": search,
             "time_includes": "device copy of the input (ms-scale) + factorization"}
 
 

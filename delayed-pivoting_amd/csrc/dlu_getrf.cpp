@@ -206,7 +206,7 @@ struct Run {
     hipEvent_t begin, end;
     double flops, bytes;  // algorithmic flops and memory bytes of the launch
   };
-  std::vector<Timed> t_update, t_update_big, t_panel, t_solve;
+  std::vector<Timed> t_update, t_update_big, t_panel, t_solve, t_search;
   hipError_t mark(std::vector<Timed> &v, hipStream_t st, double flops, double bytes = 0.0) {
     Timed t{nullptr, nullptr, flops, bytes};
     hipError_t e = hipEventCreate(&t.begin);
@@ -229,7 +229,7 @@ struct Run {
     return sec;
   }
   void release() {
-    for (auto *v : {&t_update, &t_update_big, &t_panel, &t_solve})
+    for (auto *v : {&t_update, &t_update_big, &t_panel, &t_solve, &t_search})
       for (Timed &t : *v) {
         if (t.begin) hipEventDestroy(t.begin);
         if (t.end) hipEventDestroy(t.end);
@@ -251,7 +251,7 @@ struct Run {
     return groups <= max_groups;
   }
 
-  hipError_t panel(int pb, int pe, int positioned, int parity) {
+  hipError_t panel(int pb, int pe, int positioned, int parity, hipStream_t st) {
     int groups = 0, rpl = 0;
     if (!panel_shape(pb, groups, rpl)) return hipErrorInvalidValue;
     PanelParams<T> p{};
@@ -265,8 +265,8 @@ struct Run {
     // Step tags are 16 bits wide in the exchange headers: before they would wrap, every
     // exchange word is cleared (tag 0, never a valid step) and numbering restarts at 1.
     if (w.seq + static_cast<unsigned>(nb) + 1u > kMaxStepTag) {
-      hipError_t e = hipMemsetAsync(w.d_slots, 0, slot_bytes(), ps);
-      if (e == hipSuccess) e = hipMemsetAsync(w.d_rowk, 0, rowk_bytes(), ps);
+      hipError_t e = hipMemsetAsync(w.d_slots, 0, slot_bytes(), st);
+      if (e == hipSuccess) e = hipMemsetAsync(w.d_rowk, 0, rowk_bytes(), st);
       if (e != hipSuccess) return e;
       w.seq = 0;
     }
@@ -283,12 +283,12 @@ struct Run {
     p.done = w.d_done;
     p.done_seq = ++w.done_seq;
     expect_done = p.done_seq;
-    hipError_t e = profile ? mark(t_panel, ps, 0.0) : hipSuccess;
-    if (e == hipSuccess) e = launch_panel(p, nb, rpl, ps);
-    if (e == hipSuccess && profile) e = close(t_panel, ps);
+    hipError_t e = profile ? mark(t_panel, st, 0.0) : hipSuccess;
+    if (e == hipSuccess) e = launch_panel(p, nb, rpl, st);
+    if (e == hipSuccess && profile) e = close(t_panel, st);
     if (e != hipSuccess) return e;
     ++panels;
-    return hipEventRecord(w.ev_ctl, ps);
+    return hipEventRecord(w.ev_ctl, st);
   }
 
   // The last panel's control block: spin on the sequence number the panel kernel stores into
@@ -475,7 +475,7 @@ struct Run {
   hipError_t launch_next(int pb_next, int pe_next, int positioned, int parity) {
     hipError_t e = hipEventRecord(w.ev_next, us);
     if (e == hipSuccess) e = hipStreamWaitEvent(ps, w.ev_next, 0);
-    if (e == hipSuccess) e = panel(pb_next, pe_next, positioned, (parity + 1) % kLists);
+    if (e == hipSuccess) e = panel(pb_next, pe_next, positioned, (parity + 1) % kLists, ps);
     return e;
   }
 
@@ -489,7 +489,7 @@ struct Run {
     if ((e = hipStreamWaitEvent(us, w.ev_fork, 0)) != hipSuccess) return e;
     int pb = 0, parity = 0;
     int pe = std::min(pb + nb, steps);
-    if ((e = panel(pb, pe, 0, parity)) != hipSuccess) return e;
+    if ((e = panel(pb, pe, 0, parity, ps)) != hipSuccess) return e;
     for (;;) {
       Control c;
       if ((e = wait_panel(c)) != hipSuccess) return e;
@@ -572,15 +572,22 @@ struct Run {
           if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
           if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
         }
+        if (profile &&
+            (e = mark(t_search, us, 0.0,
+                      sizeof(T) * static_cast<double>(m - stop) * static_cast<double>(n - stop))) !=
+                hipSuccess)
+          return e;
         if ((e = launch_active_search(a, lda, m, n, stop, w.d_pv, w.d_po, w.d_ctl, us)) !=
             hipSuccess)
           return e;
+        if (profile && (e = close(t_search, us)) != hipSuccess) return e;
         if ((e = launch_fallback_apply(a, lda, m, n, stop, w.d_ipiv, w.d_jpiv, w.d_ctl, us)) !=
             hipSuccess)
           return e;
         pb = stop;
         pe = std::min(pb + nb, steps);
-        if ((e = launch_next(pb, pe, 1, parity)) != hipSuccess) return e;
+        // The restarted panel follows the fallback on the same stream (nothing runs beside it).
+        if ((e = panel(pb, pe, 1, (parity + 1) % kLists, us)) != hipSuccess) return e;
       }
       parity = (parity + 1) % kLists;
       if ((e = launch_tail()) != hipSuccess) return e;
@@ -743,6 +750,9 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
       rep->panel_seconds = Run<T>::total(run.t_panel, nullptr);
       rep->solve_launches = static_cast<int>(run.t_solve.size());
       rep->solve_seconds = Run<T>::total(run.t_solve, nullptr);
+      rep->search_launches = static_cast<int>(run.t_search.size());
+      double search_flops = 0.0;
+      rep->search_seconds = Run<T>::total(run.t_search, &search_flops, &rep->search_bytes);
       unsigned long long ticks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       if (hipMemcpy(ticks, w.d_phase, sizeof ticks, hipMemcpyDeviceToHost) == hipSuccess)
         for (int i = 0; i < 8; ++i) rep->panel_phase_seconds[i] = ticks[i] * 1e-8;  // 100 MHz

@@ -46,27 +46,46 @@ __device__ __forceinline__ void block_best(double &v, unsigned long long &o) {
   }
 }
 
-// One workgroup per group of columns; the last workgroup to finish reduces the partials.
-// Magnitudes are compared in double (exact for float entries).
+// The active block is cut into items of one column x 512 rows; every wave of the grid takes
+// items in increasing order (item = column * segments + segment), each lane with 8 independent
+// coalesced loads in flight.  A lane therefore meets its elements in increasing column-major
+// offset, so a strict > keeps the lowest offset among its own equal maxima; the reductions
+// across lanes, waves and workgroups apply the full rule (larger magnitude, then lower offset).
+// The last workgroup to finish reduces the partials.  Magnitudes are compared in double (exact
+// for float entries); NaN compares false and is never selected.
 template <typename T>
 __global__ __launch_bounds__(256) void active_search_kernel(const T *__restrict__ a, int lda,
                                                            int m, int n, int k,
                                                            double *partial_v,
                                                            unsigned long long *partial_o,
                                                            Control *ctl) {
+  constexpr int U = 8, SEG = 64 * U;
   const int rows = m - k, cols = n - k;
+  const int nseg = (rows + SEG - 1) / SEG;
+  const long long items = static_cast<long long>(cols) * nseg;
+  const int lane = threadIdx.x & 63;
+  const long long waves = static_cast<long long>(gridDim.x) * 4;
   double best = 0.0;  // a zero entry never displaces the initial candidate
   unsigned long long bo = 0ull;
-  for (int col = blockIdx.x; col < cols; col += gridDim.x) {
+  for (long long it = static_cast<long long>(blockIdx.x) * 4 + (threadIdx.x >> 6); it < items;
+       it += waves) {
+    const int col = static_cast<int>(it / nseg), seg = static_cast<int>(it % nseg);
     const T *cp = a + static_cast<size_t>(k) + static_cast<size_t>(k + col) * lda;
     const unsigned long long base = static_cast<unsigned long long>(col) * rows;
-    for (int r = threadIdx.x; r < rows; r += 256) {
-      const double x = fabs(static_cast<double>(cp[r]));
-      const unsigned long long o = base + r;
-      if (better(x, o, best, bo)) {
-        best = x;
-        bo = o;
-      }
+    const int r0 = seg * SEG + lane;
+    T x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {  // clamped, unconditional loads
+      const int r = r0 + 64 * u;
+      x[u] = cp[r < rows ? r : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + 64 * u;
+      const double v = r < rows ? fabs(static_cast<double>(x[u])) : 0.0;
+      const bool take = v > best;
+      bo = take ? base + static_cast<unsigned long long>(r) : bo;
+      best = take ? v : best;
     }
   }
   block_best(best, bo);
@@ -82,13 +101,24 @@ __global__ __launch_bounds__(256) void active_search_kernel(const T *__restrict_
   __threadfence();
   best = 0.0;
   bo = 0ull;
-  for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += 256) {
-    const double v = __hip_atomic_load(&partial_v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long o =
-        __hip_atomic_load(&partial_o[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (better(v, o, best, bo)) {
-      best = v;
-      bo = o;
+  {
+    // Every partial load is issued before any is used: one memory round trip, not one per
+    // partial (at most 2048 partials, 8 per thread).
+    constexpr int PER = 2048 / 256;
+    double pv[PER];
+    unsigned long long po[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int i = threadIdx.x + 256 * j;
+      const int ic = i < static_cast<int>(gridDim.x) ? i : 0;
+      pv[j] = __hip_atomic_load(&partial_v[ic], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      po[j] = __hip_atomic_load(&partial_o[ic], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const bool take = threadIdx.x + 256 * j < gridDim.x && better(pv[j], po[j], best, bo);
+      bo = take ? po[j] : bo;
+      best = take ? pv[j] : best;
     }
   }
   __syncthreads();
@@ -101,11 +131,14 @@ __global__ __launch_bounds__(256) void active_search_kernel(const T *__restrict_
   }
 }
 
-// Step k of a full-pivot fallback, part 1: record the decision and swap rows k and best_row over
-// all n columns.  An all-zero active block marks the deferral instead (info = k).
+// Step k of a full-pivot fallback in one launch: record the decision, swap rows k and best_row
+// over all n columns, then columns k and best_col over all m rows.  The two swaps meet only in
+// the four elements of rows {k, r} x columns {k, c}; one thread does those, rows first, as the
+// reference does, and the grid does every other element (disjoint sets, no ordering needed).
+// An all-zero active block marks the deferral instead (info = k).
 template <typename T>
-__global__ __launch_bounds__(256) void fallback_rows_kernel(T *a, int lda, int n, int k,
-                                                           int *ipiv, int *jpiv, Control *ctl) {
+__global__ __launch_bounds__(256) void fallback_apply_kernel(T *a, int lda, int m, int n, int k,
+                                                            int *ipiv, int *jpiv, Control *ctl) {
   const double best = ctl->best_value;
   if (!(best > 0.0)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -114,36 +147,35 @@ __global__ __launch_bounds__(256) void fallback_rows_kernel(T *a, int lda, int n
     }
     return;
   }
-  const int r = ctl->best_row;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  const int r = ctl->best_row, c = ctl->best_col;
+  auto at = [&](int i, int j) -> T & {
+    return a[static_cast<size_t>(i) + static_cast<size_t>(j) * lda];
+  };
+  auto swap = [](T &x, T &y) {
+    const T t = x;
+    x = y;
+    y = t;
+  };
+  const int t = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
+  if (t == 0) {
     ipiv[k] = r;
-    jpiv[k] = ctl->best_col;
+    jpiv[k] = c;
     ctl->fallbacks += 1;
+    if (r != k) {
+      swap(at(k, k), at(r, k));
+      if (c != k) swap(at(k, c), at(r, c));
+    }
+    if (c != k) {
+      swap(at(k, k), at(k, c));
+      if (r != k) swap(at(r, k), at(r, c));
+    }
   }
-  if (r == k) return;
-  for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
-    T *x = a + static_cast<size_t>(k) + static_cast<size_t>(j) * lda;
-    T *y = a + static_cast<size_t>(r) + static_cast<size_t>(j) * lda;
-    const T t = *x;
-    *x = *y;
-    *y = t;
-  }
-}
-
-// Part 2: swap columns k and best_col over all m rows (after the row swap).
-template <typename T>
-__global__ __launch_bounds__(256) void fallback_cols_kernel(T *a, int lda, int m, int k,
-                                                           const Control *ctl) {
-  if (ctl->deferred || !(ctl->best_value > 0.0)) return;
-  const int c = ctl->best_col;
-  if (c == k) return;
-  T *x = a + static_cast<size_t>(k) * lda;
-  T *y = a + static_cast<size_t>(c) * lda;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
-    const T t = x[i];
-    x[i] = y[i];
-    y[i] = t;
-  }
+  if (r != k)
+    for (int j = t; j < n; j += stride)
+      if (j != k && j != c) swap(at(k, j), at(r, j));
+  if (c != k)
+    for (int i = t; i < m; i += stride)
+      if (i != k && i != r) swap(at(i, k), at(i, c));
 }
 
 // Net row movement of a panel applied to columns [c0, c1): one wave per column gathers every
@@ -192,8 +224,11 @@ hipError_t launch_init_steps(int *ipiv, int *jpiv, double *margins, int steps, h
 template <typename T>
 hipError_t launch_active_search(const T *a, int lda, int m, int n, int k, double *partial_v,
                                 unsigned long long *partial_o, Control *ctl, hipStream_t s) {
-  const int cols = n - k;
-  const int grid = cols < 2048 ? (cols > 0 ? cols : 1) : 2048;
+  // Four waves per workgroup, about four items per wave when the block is large; at most as
+  // many workgroups as there are partial slots.
+  const long long items = static_cast<long long>(n - k) * ((m - k + 511) / 512);
+  const long long want = (items + 15) / 16;
+  const int grid = want < 1 ? 1 : (want > 2048 ? 2048 : static_cast<int>(want));
   hipLaunchKernelGGL(active_search_kernel<T>, dim3(grid), dim3(256), 0, s, a, lda, m, n, k,
                      partial_v, partial_o, ctl);
   return hipGetLastError();
@@ -202,14 +237,10 @@ hipError_t launch_active_search(const T *a, int lda, int m, int n, int k, double
 template <typename T>
 hipError_t launch_fallback_apply(T *a, int lda, int m, int n, int k, int *ipiv, int *jpiv,
                                  Control *ctl, hipStream_t s) {
-  const int gr = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
-  hipLaunchKernelGGL(fallback_rows_kernel<T>, dim3(gr > 0 ? gr : 1), dim3(256), 0, s, a, lda, n,
+  const int len = m > n ? m : n;
+  const int g = (len + 255) / 256 < 1024 ? (len + 255) / 256 : 1024;
+  hipLaunchKernelGGL(fallback_apply_kernel<T>, dim3(g > 0 ? g : 1), dim3(256), 0, s, a, lda, m, n,
                      k, ipiv, jpiv, ctl);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  const int gc = (m + 255) / 256 < 1024 ? (m + 255) / 256 : 1024;
-  hipLaunchKernelGGL(fallback_cols_kernel<T>, dim3(gc > 0 ? gc : 1), dim3(256), 0, s, a, lda, m,
-                     k, ctl);
   return hipGetLastError();
 }
 

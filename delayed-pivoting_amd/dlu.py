@@ -45,7 +45,8 @@ class Report(ctypes.Structure):
                 ("panel_seconds", ctypes.c_double), ("solve_launches", ctypes.c_int),
                 ("solve_seconds", ctypes.c_double),
                 ("panel_phase_seconds", ctypes.c_double * 8),
-                ("large_update_bytes", ctypes.c_double)]
+                ("large_update_bytes", ctypes.c_double), ("search_launches", ctypes.c_int),
+                ("search_seconds", ctypes.c_double), ("search_bytes", ctypes.c_double)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}

@@ -106,6 +106,11 @@ typedef struct dlu_report {
   double panel_phase_seconds[8];
   double large_update_bytes;   /* least memory traffic of the large updates: C read and written
                                   once, A and B read once (profile only) */
+  /* full-pivot searches of the active block (profile only): launches, summed durations and the
+   * bytes they must read, sizeof(T) * (m - k) * (n - k) each */
+  int search_launches;
+  double search_seconds;
+  double search_bytes;
 } dlu_report;
 
 /* dlu_getrf_device with options and a report (either may be NULL).  `margins`, when not NULL,

@@ -198,6 +198,38 @@ def test_nan_column_is_never_a_pivot():
     assert np.isfinite(lu[:info, :info]).all()
 
 
+def test_full_pivot_search_exact_ties():
+    """Full-pivot fallback (delayed_dgetrf.cpp:32-47): among equal largest magnitudes the lowest
+    column-major offset of the active block wins.  Column 0 is zero, so step 0 searches the whole
+    1500 x 1500 block; three entries of magnitude 7.5 (above every N(0,1) entry of this matrix)
+    sit in different columns and 512-row segments, i.e. in different waves of the search."""
+    n = 1500
+    a = workloads.normal(n, n, 11)
+    assert np.abs(a).max() < 7.5
+    a[:, 0] = 0.0
+    a[1200, 9] = -7.5
+    a[1499, 3] = 7.5
+    a[700, 3] = -7.5
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a)
+    assert (int(ipiv[0]), int(jpiv[0])) == (700, 3)
+    assert rep["fallbacks"] >= 1
+    check_parity(a, lu, ipiv, jpiv, info, marg)
+
+
+def test_full_pivot_search_skips_nan():
+    """A NaN in the active block never wins the full-pivot search (NaN compares false,
+    delayed_lu_cuda/delayed_dgetrf_cuda.cu:46): the first fallback takes the largest finite
+    entry although the NaN lies at a lower offset."""
+    n = 1500
+    a = workloads.normal(n, n, 12)
+    assert np.abs(a).max() < 7.5
+    a[:, 0] = 0.0
+    a[3, 1] = np.nan
+    a[900, 6] = 7.5
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a)
+    assert (int(ipiv[0]), int(jpiv[0])) == (900, 6)
+
+
 def test_host_equals_device_and_deterministic():
     a = workloads.normal(700, 700, 21)
     lu1, ip1, jp1, info1, _, _ = gpu_factor(a, margins=False)
