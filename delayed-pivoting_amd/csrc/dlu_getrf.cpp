@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -91,8 +92,72 @@ constexpr size_t slot_bytes() {
 }
 constexpr size_t rowk_bytes() { return 2ull * rowk_words(256) * sizeof(uint64_t); }
 
-std::mutex g_lock;
-Workspace g_ws[kMaxDevices];
+// Factorization workspaces: a small pool per device.  A call leases one for its whole run, so
+// calls from several host threads (the serving workers, or a caller's own threads) overlap on
+// the device.  Admission keeps panels from blocking one another: a panel kernel needs all of its
+// workgroups resident at once (one per CU), so the panel workgroups of the calls running on a
+// device never exceed its CU count, and at most one call at a time uses a CU-partitioned panel
+// stream (every partition covers the same CUs).  Every other kernel of a factorization runs to
+// completion on its own or waits only on workgroups that already run.
+constexpr int kPoolSize = 4;
+struct DevicePool {
+  std::mutex mu;
+  std::condition_variable cv;
+  Workspace ws[kPoolSize];
+  bool busy[kPoolSize] = {};
+  int groups = 0;       // panel workgroups of the running calls
+  bool masked = false;  // a running call uses a CU-partitioned panel stream
+};
+DevicePool g_pool[kMaxDevices];
+
+class Lease {
+ public:
+  Lease(int device, int groups, bool masked, int cus)
+      : p_(g_pool[device]), groups_(groups), masked_(masked) {
+    std::unique_lock<std::mutex> lk(p_.mu);
+    p_.cv.wait(lk, [&] {
+      bool free = false;
+      for (bool b : p_.busy) free = free || !b;
+      return free && (p_.groups == 0 || p_.groups + groups <= cus) && !(masked && p_.masked);
+    });
+    for (idx_ = 0; p_.busy[idx_]; ++idx_) {
+    }
+    p_.busy[idx_] = true;
+    p_.groups += groups;
+    p_.masked = p_.masked || masked;
+  }
+  ~Lease() {
+    {
+      std::lock_guard<std::mutex> lk(p_.mu);
+      p_.busy[idx_] = false;
+      p_.groups -= groups_;
+      if (masked_) p_.masked = false;
+    }
+    p_.cv.notify_all();
+  }
+  Lease(const Lease &) = delete;
+  Lease &operator=(const Lease &) = delete;
+  Workspace &workspace() { return p_.ws[idx_]; }
+
+ private:
+  DevicePool &p_;
+  int groups_;
+  bool masked_;
+  int idx_ = 0;
+};
+
+// Rows per lane and workgroups of the panel kernel for `rows` rows: the panel is held in
+// registers, so as few workgroups as possible (the most rows per lane the panel width allows).
+inline void panel_shape_for(int rows, int nb, int forced_rpl, int max_groups, int &groups,
+                            int &rpl) {
+  if (forced_rpl > 0) {
+    rpl = forced_rpl;
+  } else {
+    rpl = nb == 256 ? 2 : 4;
+    if (nb == 256 && (rows + 63) / 64 <= max_groups) rpl = 1;  // avoid the spilling variant
+  }
+  groups = (rows + 64 * rpl - 1) / (64 * rpl);
+}
 
 hipError_t prepare(Workspace &w, int device, int steps) {
   if (!w.ready) {
@@ -240,14 +305,7 @@ struct Run {
   // The panel is held in registers: as few workgroups as possible (the most rows per lane the
   // panel width allows), and no more than the panel stream can keep resident together.
   bool panel_shape(int pb, int &groups, int &rpl) const {
-    const int rows = m - pb;
-    if (forced_rpl > 0) {
-      rpl = forced_rpl;
-    } else {
-      rpl = nb == 256 ? 2 : 4;
-      if (nb == 256 && (rows + 63) / 64 <= max_groups) rpl = 1;  // avoid the spilling variant
-    }
-    groups = (rows + 64 * rpl - 1) / (64 * rpl);
+    panel_shape_for(m - pb, nb, forced_rpl, max_groups, groups, rpl);
     return groups <= max_groups;
   }
 
@@ -645,39 +703,50 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
     *info = -1;
     return DLU_RUNTIME_FAILURE;
   }
-  std::lock_guard<std::mutex> guard(g_lock);
-  Workspace &w = g_ws[device];
+  int forced_rpl = 0;
+  if (opt && opt->rows_per_lane) {
+    if (opt->rows_per_lane != 1 && opt->rows_per_lane != 2 && opt->rows_per_lane != 4) {
+      *info = -1;
+      return DLU_INVALID_ARGUMENT;
+    }
+    forced_rpl = opt->rows_per_lane;
+  }
+  int cus = 0;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) !=
+          hipSuccess ||
+      cus <= 0) {
+    *info = -1;
+    return DLU_RUNTIME_FAILURE;
+  }
+  // CUs reserved for the panel stream.  FP64: none by default (measured: the trailing update
+  // gains more from the whole device than the panels lose by waiting for update workgroups to
+  // drain from a CU).  FP32, whose trailing update is twice as fast, is panel-bound: above 16
+  // panel workgroups (more than 4096 rows), enough CUs for one resident workgroup per panel row
+  // block (at most half the device), which also lets the far GEMM's tail run on them behind each
+  // panel; smaller factorizations keep the whole device, so several of them (serving) overlap.
+  // opt->panel_cus > 0 forces a size, < 0 disables.
+  int groups = 0, rpl = 0;
+  panel_shape_for(m, nb, forced_rpl, std::min(kMaxPanelGroups, cus), groups, rpl);
+  int want = 0;
+  if (sizeof(T) == 4 && groups > 16)
+    want = std::max(16, std::min(((groups + 15) / 16) * 16, cus / 2));
+  if (opt && opt->panel_cus > 0) want = std::min(opt->panel_cus, cus);
+  if (opt && opt->panel_cus < 0) want = 0;
+  if (groups > want && want > 0) want = 0;  // too tall for a partition: share the device
+  Lease lease(device, std::min(groups, cus), want > 0, cus);
+  Workspace &w = lease.workspace();
   if ((e = prepare(w, device, steps)) != hipSuccess) {
     *info = -1;
     return to_status(e);
   }
   Run<T> run{w, m, n, lda, steps, nb, d_a, stream, spin};
   run.profile = opt && opt->profile;
-  if (opt && opt->rows_per_lane) {
-    if (opt->rows_per_lane != 1 && opt->rows_per_lane != 2 && opt->rows_per_lane != 4) {
-      *info = -1;
-      return DLU_INVALID_ARGUMENT;
-    }
-    run.forced_rpl = opt->rows_per_lane;
-  }
+  run.forced_rpl = forced_rpl;
   if (opt && opt->merge_rows != 0) run.merge_rows = opt->merge_rows < 0 ? (1 << 30) : opt->merge_rows;
   run.split = !(opt && opt->split_update < 0);
   if (opt && opt->split_update > 0) run.forced_split = opt->split_update;
   {
-    // CUs reserved for the panel stream.  FP64: none by default (measured: the trailing update
-    // gains more from the whole device than the panels lose by waiting for update workgroups to
-    // drain from a CU).  FP32, whose trailing update is twice as fast, is panel-bound: enough
-    // CUs for one resident workgroup per panel row block (at most half the device), which also
-    // lets the far GEMM's tail run on them behind each panel.  opt->panel_cus > 0 forces a
-    // size, < 0 disables.
-    int groups = 0, rpl = 0;
     run.max_groups = std::min(kMaxPanelGroups, w.cus);
-    run.panel_shape(0, groups, rpl);
-    int want = 0;
-    if (sizeof(T) == 4) want = std::max(16, std::min(((groups + 15) / 16) * 16, w.cus / 2));
-    if (opt && opt->panel_cus > 0) want = std::min(opt->panel_cus, w.cus);
-    if (opt && opt->panel_cus < 0) want = 0;
-    if (groups > want && want > 0) want = 0;  // too tall for a partition: share the device
     Workspace::Streams *st = nullptr;
     if ((e = stream_pair(w, want, st)) != hipSuccess) {
       *info = -1;

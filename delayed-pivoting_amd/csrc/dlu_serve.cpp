@@ -1,13 +1,15 @@
 // The solve service: continuous batching of factor + solve requests on one device.
 //
-// Two workers share the device.  The batch worker packs every queued request with n <= cutoff
-// into one launch of the one-workgroup-per-matrix kernel (dlu_batch.hip) per round; requests
-// that arrive while a round runs wait for the next round.  The full worker takes the larger
-// requests one at a time through the blocked factorization (FP64, or FP32 + FP64 refinement).
+// Workers share the device.  The batch worker packs every queued request with n <= cutoff into
+// one launch of the one-workgroup-per-matrix kernel (dlu_batch.hip) per round; requests that
+// arrive while a round runs wait for the next round.  The full workers (large_workers of them,
+// each with its own stream) take the larger requests through the blocked factorization (FP64,
+// or FP32 + FP64 refinement); their factorizations overlap on the device, since a factorization
+// of serving size keeps only a few CUs busy with its panel.
 // Memory for factors and per-request buffers comes from the stream-ordered allocator and is
 // admitted against an HBM budget: kept factors stay resident until an admission needs their
 // memory (least recently used first, never while a request is using them).  A worker blocks on
-// memory only while it holds none, so the two workers cannot wait on each other.
+// memory only while it holds none, so the workers cannot wait on each other.
 #include "dlu.h"
 #include "dlu_internal.hpp"
 
@@ -67,10 +69,12 @@ struct Server {
   unsigned long long tick = 0;
   long long inflight = 0;
   bool stop = false;
-  hipStream_t s_small = nullptr, s_large = nullptr;
+  hipStream_t s_small = nullptr;
+  std::vector<hipStream_t> s_large;  // one per full worker
   BatchItem *h_items = nullptr, *d_items = nullptr;
   int *h_infos = nullptr, *d_infos = nullptr;
-  std::thread t_small, t_large;
+  std::thread t_small;
+  std::vector<std::thread> t_large;
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
@@ -273,7 +277,7 @@ struct Server {
   }
 
   // ---- the full-factorization worker -------------------------------------------------------
-  void large_loop() {
+  void large_loop(hipStream_t s_large) {
     hipSetDevice(device);
     for (;;) {
       Job j;
@@ -459,7 +463,6 @@ dlu_server *dlu_server_create(const dlu_server_config *config) {
   s->budget = static_cast<size_t>(s->cfg.hbm_budget_gib * 1073741824.0);
   if (hipGetDevice(&s->device) != hipSuccess ||
       hipStreamCreateWithFlags(&s->s_small, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&s->s_large, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc(&s->h_items, sizeof(dlu::BatchItem) * s->cfg.max_batch) != hipSuccess ||
       hipMalloc(&s->d_items, sizeof(dlu::BatchItem) * s->cfg.max_batch) != hipSuccess ||
       hipHostMalloc(&s->h_infos, sizeof(int) * s->cfg.max_batch) != hipSuccess ||
@@ -469,8 +472,23 @@ dlu_server *dlu_server_create(const dlu_server_config *config) {
   }
   s->cutoff = s->cfg.batch_cutoff > 0 ? std::min(s->cfg.batch_cutoff, DLU_BATCH_MAX_N)
                                       : s->tune_cutoff();
+  const int workers = s->cfg.large_workers > 0 ? std::min(s->cfg.large_workers, 4) : 3;
+  for (int i = 0; i < workers; ++i) {
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) break;
+    s->s_large.push_back(st);
+  }
+  if (s->s_large.empty()) {
+    hipFree(s->d_items);
+    hipFree(s->d_infos);
+    hipHostFree(s->h_items);
+    hipHostFree(s->h_infos);
+    hipStreamDestroy(s->s_small);
+    delete s;
+    return nullptr;
+  }
   s->t_small = std::thread([s] { s->small_loop(); });
-  s->t_large = std::thread([s] { s->large_loop(); });
+  for (hipStream_t st : s->s_large) s->t_large.emplace_back([s, st] { s->large_loop(st); });
   return s;
 }
 
@@ -520,20 +538,21 @@ void dlu_server_destroy(dlu_server *s) {
   s->cv_large.notify_all();
   s->cv_mem.notify_all();
   if (s->t_small.joinable()) s->t_small.join();
-  if (s->t_large.joinable()) s->t_large.join();
+  for (std::thread &t : s->t_large)
+    if (t.joinable()) t.join();
   for (auto &kv : s->store) {
     hipFreeAsync(kv.second->mem, kv.second->stream);
     delete kv.second;
   }
   s->store.clear();
   hipStreamSynchronize(s->s_small);
-  hipStreamSynchronize(s->s_large);
+  for (hipStream_t st : s->s_large) hipStreamSynchronize(st);
   hipFree(s->d_items);
   hipFree(s->d_infos);
   hipHostFree(s->h_items);
   hipHostFree(s->h_infos);
   hipStreamDestroy(s->s_small);
-  hipStreamDestroy(s->s_large);
+  for (hipStream_t st : s->s_large) hipStreamDestroy(st);
   delete s;
 }
 

@@ -79,7 +79,8 @@ class BatchItem(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_gib", ctypes.c_double), ("batch_cutoff", ctypes.c_int),
-                ("mixed", ctypes.c_int), ("tol", ctypes.c_double), ("max_batch", ctypes.c_int)]
+                ("mixed", ctypes.c_int), ("tol", ctypes.c_double), ("max_batch", ctypes.c_int),
+                ("large_workers", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -183,8 +184,10 @@ def batch_gesv_device(items, stream=None) -> Status:
 class Server:
     """Thin wrapper of dlu_server_*: submit requests, poll completions, drain, close."""
 
-    def __init__(self, budget_gib=32.0, batch_cutoff=0, mixed=False, tol=1e-13, max_batch=1024):
-        cfg = ServerConfig(budget_gib, batch_cutoff, 1 if mixed else 0, tol, max_batch)
+    def __init__(self, budget_gib=32.0, batch_cutoff=0, mixed=False, tol=1e-13, max_batch=1024,
+                 large_workers=0):
+        cfg = ServerConfig(budget_gib, batch_cutoff, 1 if mixed else 0, tol, max_batch,
+                           large_workers)
         self._h = lib().dlu_server_create(ctypes.byref(cfg))
         if not self._h:
             raise RuntimeError("dlu_server_create failed")

@@ -36,7 +36,9 @@ typedef enum dlu_status {
 } dlu_status;
 
 /* Replaces delayed_getrf_cuda_device (delayed_lu_cuda/delayed_dgetrf_cuda.hpp:24-26,
- * delayed_dgetrf_cuda.cu:403-447).  d_a is device memory. */
+ * delayed_dgetrf_cuda.cu:403-447).  d_a is device memory.  Calls from several host threads on
+ * one device run concurrently (up to 4 at a time, fewer while their panels would not fit on the
+ * CUs together; further calls wait), as the reference's calls on different streams do. */
 dlu_status dlu_getrf_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv, int *info,
                             void *stream);
 
@@ -71,8 +73,8 @@ typedef struct dlu_options {
   long long spin_limit_ticks; /* bound on any in-kernel wait, 100 MHz ticks (default 2e8 = 2 s) */
   int profile;     /* nonzero: time every panel / solve / update launch with HIP events */
   int panel_cus;   /* CUs reserved for panel kernels while updates run on the rest: > 0 = that
-                      many, < 0 = none, 0 = automatic (none for FP64; for FP32 one CU per panel
-                      workgroup, at most half the device) */
+                      many, < 0 = none, 0 = automatic (none for FP64; for FP32 above 4096 rows,
+                      one CU per panel workgroup, at most half the device) */
   int rows_per_lane; /* panel rows held per lane: 0 = automatic, else 1, 2 or 4 */
   int merge_rows;    /* pairs of panels share one trailing GEMM (K = 2 nb) while at least this
                         many rows remain below them: 0 = automatic (3000), < 0 = never */
@@ -228,6 +230,8 @@ typedef struct dlu_server_config {
   int mixed;             /* 1: FP32 factor + FP64 refinement above the cutoff; 0: FP64 */
   double tol;            /* refinement target backward error (default 1e-13) */
   int max_batch;         /* most requests per batched launch (default 1024) */
+  int large_workers;     /* requests above the cutoff factored at once, each on its own stream
+                            (default 3, at most 4) */
 } dlu_server_config;
 
 typedef struct dlu_request {
