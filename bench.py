@@ -10,9 +10,11 @@ Also reported on the same JSON line:
   roofline      for the dominant kernel (the FP64 MFMA trailing update): algorithmic flops per
                 launch / mean launch duration, both measured live with HIP events on the stream
                 each launch runs on (one profiled factorization after the timed steps);
-  cpu_baseline  the oracle (C restatement of the reference delayed_getrf on the host BLAS) on a
-                bounded sample, plus LAPACK dgetrf on the same sample;
-  parity        permutations of the first timed factorization versus the oracle at --verify n.
+  cpu_baseline  the oracle (C restatement of the reference delayed_getrf on the host BLAS) on the
+                same n = 16384 matrix, timed on the host threads one GPU gets, plus LAPACK
+                dgetrf on it;
+  parity        permutations (bit for bit) and factors (relative Frobenius difference) of the
+                timed factorization versus the oracle's.
 """
 import argparse
 import importlib.util
@@ -47,29 +49,44 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n_sample, threads):
+def cpu_and_parity(a, lu_gpu, first, threads):
+    """The oracle (C restatement of the reference delayed_getrf, nb = 64, on the host BLAS) and
+    LAPACK dgetrf on the full benchmark matrix, each timed once on `threads` host threads: the
+    CPU baseline, and the parity check of the timed factorization (permutations bit for bit,
+    factors within 1e-12 relative)."""
     import numpy as np
     oracle = _load("dlu_oracle_py", os.path.join(ROOT, "oracle", "oracle.py"))
-    workloads = _load("dlu_workloads", os.path.join(PKG, "workloads.py"))
     oracle.set_threads(threads)
-    a = workloads.normal(n_sample, n_sample, 7)
-    flops = 2.0 * n_sample ** 3 / 3.0
+    n = a.shape[0]
+    flops = 2.0 * n ** 3 / 3.0
     t0 = time.perf_counter()
-    _, ipiv, jpiv, info, ex = oracle.delayed_getrf(a, nb=64)
+    lu_c, ip_c, jp_c, info_c, ex = oracle.delayed_getrf(a, nb=64, margins=True)
     t_delayed = time.perf_counter() - t0
     t0 = time.perf_counter()
     _, ipl, infol = oracle.lapack_getrf(a)
     t_lapack = time.perf_counter() - t0
-    return {
+    ipiv, jpiv, rep = first
+    same = bool(np.array_equal(ipiv, ip_c) and np.array_equal(jpiv, jp_c) and info_c == n)
+    parity = {"n": n, "bit_exact_permutations": same,
+              "factor_rel_diff_fro": float(np.linalg.norm(lu_gpu - lu_c) / np.linalg.norm(lu_c)),
+              "cpu_min_margin": float(ex["margins"].min()), "gpu_min_margin": rep["min_margin"],
+              "gpu_near_ties": rep["near_ties"], "gpu_exact_ties": rep["exact_ties"]}
+    if not same:
+        k = int(np.argmax((ipiv != ip_c) | (jpiv != jp_c)))
+        parity["first_differing_step"] = k
+        parity["cpu_margin_at_first_difference"] = float(ex["margins"][k])
+    cpu = {
         "value": round(flops / t_delayed / 1e12, 4), "unit": "TFLOP/s", "cores": threads,
         "kind": "port",
-        "sample": f"one delayed_getrf restatement (nb=64) on a {n_sample}x{n_sample} N(0,1) "
-                  f"matrix, {oracle.blas_name().strip()[:60]}, {threads} threads",
-        "seconds": round(t_delayed, 3), "info": info,
+        "sample": f"the whole workload: one delayed_getrf restatement (nb=64) on the same "
+                  f"{n}x{n} N(0,1) matrix, {oracle.blas_name().strip()[:60]}, {threads} threads "
+                  f"(the CPUs this host gives one GPU)",
+        "seconds": round(t_delayed, 3), "info": info_c,
         "lapack_dgetrf": {"value": round(flops / t_lapack / 1e12, 4), "unit": "TFLOP/s",
                           "seconds": round(t_lapack, 3),
-                          "same_ipiv": bool(np.array_equal(ipiv, ipl))},
+                          "same_ipiv": bool(np.array_equal(ip_c, ipl))},
     }
+    return cpu, parity
 
 
 def _timed(torch, fn, reps=2):
@@ -86,31 +103,37 @@ def _timed(torch, fn, reps=2):
     return best
 
 
-def bench_delay_stress(dlu, torch, dev, n):
-    """Delay-stress FP64: 5% zero columns, 5% scaled by 1e-14, a rank-32 64x64 diagonal block."""
+def bench_delay_stress(dlu, torch, dev, n, column=False):
+    """Delay-stress FP64: 5% zero columns, 5% scaled by 1e-14, a rank-32 64x64 diagonal block;
+    the full-pivot fallback (delayed_getrf) or the column-delay variant (delayed_getrf_column)."""
     import numpy as np
     workloads = _load("dlu_workloads", os.path.join(PKG, "workloads.py"))
-    log(f"delay-stress n={n}")
+    log(f"delay-stress n={n} column={column}")
     a, zero_cols, _, _ = workloads.delay_stress(n, n)
     orig = torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
     work = torch.empty_like(orig)
+    fb = dlu.FALLBACK_COLUMN if column else dlu.FALLBACK_FULL
 
     def run():
         work.copy_(orig)
-        return dlu.getrf_device(n, n, work.data_ptr(), n, options={"nb": 0})
+        return dlu.getrf_device(n, n, work.data_ptr(), n, options={"nb": 0, "fallback": fb})
 
     t, (st, ipiv, jpiv, info, rep, _) = _timed(torch, run)
     delayed = np.sort(dlu.net_order(jpiv, info, n)[info:])
     # One more, profiled run: the full-pivot searches' own time and the bytes they must read.
     work.copy_(orig)
-    _, _, _, _, prep, _ = dlu.getrf_device(n, n, work.data_ptr(), n, options={"nb": 0, "profile": 1})
+    _, _, _, _, prep, _ = dlu.getrf_device(n, n, work.data_ptr(), n,
+                                           options={"nb": 0, "profile": 1, "fallback": fb})
     torch.cuda.synchronize()
     search = {"launches": prep["search_launches"], "ms": round(prep["search_seconds"] * 1e3, 2),
               "bytes": prep["search_bytes"],
               "gb_per_s": round(prep["search_bytes"] / max(prep["search_seconds"], 1e-12) / 1e9, 1),
               "materialize_and_update_ms": round(prep["update_seconds"] * 1e3, 2),
               "panel_ms": round(prep["panel_seconds"] * 1e3, 2)}
-    return {"workload": f"delay-stress FP64 n={n} (5% zero cols, 5% x1e-14, rank-32 64x64 block)",
+    variant = "delayed_getrf_column (rightmost nonzero column)" if column else \
+        "delayed_getrf (full-pivot search)"
+    return {"workload": f"delay-stress FP64 n={n} (5% zero cols, 5% x1e-14, rank-32 64x64 block), "
+                        f"{variant}",
             "ms": round(t * 1e3, 2), "tflops_2n3_3": round(2.0 * n ** 3 / 3 / t / 1e12, 3),
             "info": info, "expected_info": n - len(zero_cols),
             "delayed_set_equals_zero_columns": bool(info == n - len(zero_cols) and
@@ -281,11 +304,10 @@ def main():
     ap.add_argument("--seed", type=int, default=16384)
     ap.add_argument("--panel-cus", type=int, default=0,
                     help="CUs reserved for panel kernels (0 auto, -1 none)")
-    ap.add_argument("--cpu-sample", type=int, default=8192)
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--verify", type=int, default=0,
-                    help="check permutations against the oracle on the full matrix")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline (0: OMP_NUM_THREADS, else up to 16)")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline and the parity check against the oracle")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the delay-stress, mixed-precision and serving measurements")
     ap.add_argument("--mixed-n", type=int, default=16384)
@@ -380,6 +402,7 @@ def main():
 
     # One profiled factorization: per-launch HIP events on each kernel's own stream.
     _, _, _, prep = step(profile=True)
+    torch.cuda.synchronize()
     # dominant kernel: the trailing-update GEMM (launches over at least 512 128 x 128 blocks of C)
     upd_launch_s = prep["large_update_seconds"] / max(1, prep["large_update_launches"])
     upd_flops = prep["large_update_flops"] / max(1, prep["large_update_launches"])
@@ -403,24 +426,18 @@ def main():
               "solve_ms_total": round(prep["solve_seconds"] * 1e3, 3),
               "update_ms_total": round(prep["update_seconds"] * 1e3, 3)}
 
-    parity = None
-    if args.verify:
-        oracle = _load("dlu_oracle_py", os.path.join(ROOT, "oracle", "oracle.py"))
-        a = orig.cpu().numpy().T  # column-major view of the same matrix
-        log("verifying permutations against the oracle (full matrix)")
-        _, ip_c, jp_c, info_c, ex = oracle.delayed_getrf(np.asfortranarray(a), nb=64,
-                                                         margins=True)
-        ipiv, jpiv, rep = first
-        same = bool(np.array_equal(ipiv, ip_c) and np.array_equal(jpiv, jp_c) and info_c == n)
-        parity = {"n": n, "bit_exact_permutations": same,
-                  "cpu_min_margin": float(ex["margins"].min()),
-                  "gpu_min_margin": rep["min_margin"]}
-
-    cpu = None
+    parity = cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: oracle on {args.cpu_sample}^2 with {threads} threads")
-        cpu = cpu_baseline(args.cpu_sample, threads)
+        # The last (profiled) step left the factors in `work`; the factorization is deterministic,
+        # so they are those of every timed step.
+        a = np.asfortranarray(orig.cpu().numpy().T)  # column-major view of the same matrix
+        lu_gpu = np.asfortranarray(work.cpu().numpy().T)
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
+            min(16, os.cpu_count() or 1)
+        log(f"cpu baseline and parity: oracle and LAPACK dgetrf on the {n}^2 matrix, "
+            f"{threads} threads")
+        cpu, parity = cpu_and_parity(a, lu_gpu, first, threads)
+        del a, lu_gpu
 
     del orig, work
     torch.cuda.empty_cache()
@@ -428,6 +445,7 @@ def main():
     if not args.no_extra:
         sweep = bench_sweep(dlu, torch, dev, [2048, 4096, 8192, 16384, 32768])
         c3 = bench_delay_stress(dlu, torch, dev, 8192)
+        c3["column_variant"] = bench_delay_stress(dlu, torch, dev, 8192, column=True)
         c4 = bench_mixed(dlu, torch, dev, args.mixed_n, 16)
         c5 = bench_serving(dlu, torch, dev, args.serve_requests)
 

@@ -76,12 +76,14 @@ struct Workspace {
   };
   Streams streams[4];
   int nstreams = 0;
+  int next_replace = 0;  // slot replaced when all four hold other partitions
   hipEvent_t ev_fork = nullptr, ev_ctl = nullptr, ev_next = nullptr, ev_join = nullptr;
   hipEvent_t ev_solved = nullptr, ev_tail = nullptr;  // far U block row done / tail GEMM done
   bool ready = false;
 };
 
 constexpr int kMaxDevices = 64;
+constexpr int kCusPerXcd = 32;  // MI355X (gfx950): 8 XCDs of 32 CUs
 constexpr int kSearchPartials = 2048;
 constexpr int kListStride = 4 * 256 + 4;  // swap_list_ints(256), enough for every nb
 constexpr int kLists = 4;  // panels j and j+4 share a list (a pair's far update reads two)
@@ -212,11 +214,19 @@ hipError_t stream_pair(Workspace &w, int panel_cus, Workspace::Streams *&out) {
       out = &w.streams[i];
       return hipSuccess;
     }
-  if (w.nstreams == 4) {
-    out = &w.streams[0];
-    return hipSuccess;
+  // All slots taken by other partitions: replace one (round robin).  The workspace is leased by
+  // this call alone and every earlier call joined its streams before returning, so the pair
+  // replaced holds no work.
+  int slot = w.nstreams;
+  if (slot == 4) {
+    slot = w.next_replace;
+    w.next_replace = (w.next_replace + 1) % 4;
+    Workspace::Streams &old = w.streams[slot];
+    if (old.panel) hipStreamDestroy(old.panel);
+    if (old.update) hipStreamDestroy(old.update);
+    old = Workspace::Streams{};
   }
-  Workspace::Streams &st = w.streams[w.nstreams];
+  Workspace::Streams &st = w.streams[slot];
   hipError_t e = hipSuccess;
   bool masked = false;
   if (panel_cus > 0 && panel_cus < w.cus) {
@@ -237,7 +247,7 @@ hipError_t stream_pair(Workspace &w, int panel_cus, Workspace::Streams *&out) {
     if ((e = hipStreamCreateWithFlags(&st.update, hipStreamNonBlocking)) != hipSuccess) return e;
   }
   st.panel_cus = masked ? panel_cus : 0;
-  ++w.nstreams;
+  if (slot == w.nstreams) ++w.nstreams;
   out = &st;
   return hipSuccess;
 }
@@ -264,6 +274,7 @@ struct Run {
   bool split = true;                 // tail of the far GEMM on the panel stream's CUs
   int forced_split = 0;              // > 0: that many far columns always go to the panel stream
   int panel_cus = 0;                 // CUs of the panel stream (0: no partition)
+  int variant = DLU_FALLBACK_FULL;   // fallback search: full pivot or rightmost nonzero column
   hipStream_t ps = nullptr, us = nullptr;  // internal panel / update streams
 
   // Opt-in launch timing: an event pair around each launch, on the launch's own stream.
@@ -308,10 +319,38 @@ struct Run {
     panel_shape_for(m - pb, nb, forced_rpl, max_groups, groups, rpl);
     return groups <= max_groups;
   }
+  // A panel starting at pb with more rows than the register-resident kernel can hold on the
+  // panel stream's CUs is factored by the tall path instead.
+  bool tall(int pb) const {
+    int groups = 0, rpl = 0;
+    return pb < steps && !panel_shape(pb, groups, rpl);
+  }
+
+  // Tall panel: one column step at a time (column argmax, whole-row interchange, multipliers and
+  // rank-1 update), every launch predicated on the control block; rows are interchanged across
+  // all columns at once, so its swap list is the identity.  Runs in order on `st`.
+  hipError_t tall_panel(int pb, int pe, int positioned, int parity, hipStream_t st) {
+    int *list = w.d_lists + parity * kListStride;
+    hipError_t e = profile ? mark(t_panel, st, 0.0) : hipSuccess;
+    if (e == hipSuccess) e = launch_tall_begin(w.d_ctl, pb, pe, st);
+    for (int k = pb; e == hipSuccess && k < pe; ++k) {
+      if (!(positioned && k == pb)) {
+        e = launch_active_search(a, lda, m, k + 1, k, w.d_pv, w.d_po, w.d_ctl, st);
+        if (e == hipSuccess) e = launch_tall_pivot(a, lda, n, k, w.d_ipiv, w.d_ctl, st);
+      }
+      if (e == hipSuccess) e = launch_tall_update(a, lda, m, k, pe, w.d_ctl, st);
+    }
+    expect_done = ++w.done_seq;
+    if (e == hipSuccess) e = launch_tall_finish(w.d_ctl, list, nb, pb, w.d_done, expect_done, st);
+    if (e == hipSuccess && profile) e = close(t_panel, st);
+    if (e != hipSuccess) return e;
+    ++panels;
+    return hipEventRecord(w.ev_ctl, st);
+  }
 
   hipError_t panel(int pb, int pe, int positioned, int parity, hipStream_t st) {
     int groups = 0, rpl = 0;
-    if (!panel_shape(pb, groups, rpl)) return hipErrorInvalidValue;
+    if (!panel_shape(pb, groups, rpl)) return tall_panel(pb, pe, positioned, parity, st);
     PanelParams<T> p{};
     p.a = a;
     p.lda = lda;
@@ -531,6 +570,7 @@ struct Run {
   // as soon as that block's update is queued: the far work of the step is queued after it, so
   // the host's launch calls for the far work are off the panel chain.
   hipError_t launch_next(int pb_next, int pe_next, int positioned, int parity) {
+    if (tall(pb_next)) return panel(pb_next, pe_next, positioned, (parity + 1) % kLists, us);
     hipError_t e = hipEventRecord(w.ev_next, us);
     if (e == hipSuccess) e = hipStreamWaitEvent(ps, w.ev_next, 0);
     if (e == hipSuccess) e = panel(pb_next, pe_next, positioned, (parity + 1) % kLists, ps);
@@ -547,7 +587,7 @@ struct Run {
     if ((e = hipStreamWaitEvent(us, w.ev_fork, 0)) != hipSuccess) return e;
     int pb = 0, parity = 0;
     int pe = std::min(pb + nb, steps);
-    if ((e = panel(pb, pe, 0, parity, ps)) != hipSuccess) return e;
+    if ((e = panel(pb, pe, 0, parity, tall(pb) ? us : ps)) != hipSuccess) return e;
     for (;;) {
       Control c;
       if ((e = wait_panel(c)) != hipSuccess) return e;
@@ -558,7 +598,9 @@ struct Run {
       const int pe2 = std::min(pe + nb, steps);
       const bool full = !c.deferred && !c.zero_column && wdt == nb && pe - pb == nb;
       if ((e = hipStreamWaitEvent(us, w.ev_ctl, 0)) != hipSuccess) return e;
-      const bool second = pair.open && full && pe < steps && lookahead;
+      // A tall next panel interchanges whole rows as it goes: nothing may run beside it.
+      const bool ahead = lookahead && !tall(pe);
+      const bool second = pair.open && full && pe < steps && ahead;
       if (!second && (e = close_pair(us)) != hipSuccess) return e;
       if (c.deferred) {
         if ((e = queue_rest(0)) != hipSuccess) return e;
@@ -582,7 +624,7 @@ struct Run {
           const int cm = std::min(n, pe2 + half);
           if ((e = pair_update(b0, pair.l0, list, pe2, cm, us)) != hipSuccess) return e;
           rest = Rest{true, b0, cm, pair.l0, list};
-        } else if (pe < steps && lookahead && full && pe2 < n && m - pe2 >= merge_rows) {
+        } else if (pe < steps && ahead && full && pe2 < n && m - pe2 >= merge_rows) {
           // First panel of a pair: only the next panel's columns now (and the previous pair's
           // right half, which the next panel's block does not touch).
           if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
@@ -592,7 +634,7 @@ struct Run {
           pair.b0 = pb;
           pair.fe = pe2;
           pair.l0 = list;
-        } else if (pe < steps && lookahead) {
+        } else if (pe < steps && ahead) {
           // The next panel's columns first, then the far columns while the next panel runs
           // (their last columns on the panel stream behind it when the GEMM outlasts it).
           if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
@@ -635,17 +677,25 @@ struct Run {
                       sizeof(T) * static_cast<double>(m - stop) * static_cast<double>(n - stop))) !=
                 hipSuccess)
           return e;
-        if ((e = launch_active_search(a, lda, m, n, stop, w.d_pv, w.d_po, w.d_ctl, us)) !=
-            hipSuccess)
-          return e;
+        e = variant == DLU_FALLBACK_COLUMN
+                ? launch_column_search(a, lda, m, n, stop, w.d_ctl, us)
+                : launch_active_search(a, lda, m, n, stop, w.d_pv, w.d_po, w.d_ctl, us);
+        if (e != hipSuccess) return e;
         if (profile && (e = close(t_search, us)) != hipSuccess) return e;
         if ((e = launch_fallback_apply(a, lda, m, n, stop, w.d_ipiv, w.d_jpiv, w.d_ctl, us)) !=
             hipSuccess)
           return e;
         pb = stop;
         pe = std::min(pb + nb, steps);
-        // The restarted panel follows the fallback on the same stream (nothing runs beside it).
-        if ((e = panel(pb, pe, 1, (parity + 1) % kLists, us)) != hipSuccess) return e;
+        // The restarted panel follows the fallback (nothing runs beside it): on the same stream,
+        // or, when the panel stream owns a CU partition sized for panels, there, behind it.
+        if (panel_cus > 0 && !tall(pb)) {
+          if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
+          if ((e = hipStreamWaitEvent(ps, w.ev_next, 0)) != hipSuccess) return e;
+          if ((e = panel(pb, pe, 1, (parity + 1) % kLists, ps)) != hipSuccess) return e;
+        } else if ((e = panel(pb, pe, 1, (parity + 1) % kLists, us)) != hipSuccess) {
+          return e;
+        }
       }
       parity = (parity + 1) % kLists;
       if ((e = launch_tail()) != hipSuccess) return e;
@@ -663,7 +713,44 @@ struct Run {
   }
 };
 
+// Every device resource of a workspace (streams, events, device and pinned host memory).
+void release_workspace(Workspace &w) {
+  if (!w.ready && w.d_ipiv == nullptr) return;
+  for (int i = 0; i < w.nstreams; ++i) {
+    if (w.streams[i].panel) hipStreamDestroy(w.streams[i].panel);
+    if (w.streams[i].update) hipStreamDestroy(w.streams[i].update);
+  }
+  for (hipEvent_t ev : {w.ev_fork, w.ev_ctl, w.ev_next, w.ev_join, w.ev_solved, w.ev_tail})
+    if (ev) hipEventDestroy(ev);
+  for (void *p : {static_cast<void *>(w.d_ctl), static_cast<void *>(w.d_ipiv),
+                  static_cast<void *>(w.d_jpiv), static_cast<void *>(w.d_margins),
+                  static_cast<void *>(w.d_slots), static_cast<void *>(w.d_rowk),
+                  static_cast<void *>(w.d_lists), static_cast<void *>(w.d_pv),
+                  static_cast<void *>(w.d_po), static_cast<void *>(w.d_phase)})
+    if (p) hipFree(p);
+  if (w.h_ctl) hipHostFree(w.h_ctl);
+  if (w.h_done) hipHostFree(w.h_done);
+  w = Workspace{};
+}
+
 }  // namespace
+
+// Frees the idle workspaces of every device (busy ones belong to calls still running).
+void release_pools() {
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) return;
+  for (int d = 0; d < kMaxDevices; ++d) {
+    DevicePool &p = g_pool[d];
+    std::lock_guard<std::mutex> lk(p.mu);
+    for (int i = 0; i < kPoolSize; ++i) {
+      Workspace &w = p.ws[i];
+      if (p.busy[i] || (!w.ready && w.d_ipiv == nullptr)) continue;
+      if (hipSetDevice(w.device >= 0 ? w.device : d) != hipSuccess) continue;
+      release_workspace(w);
+    }
+  }
+  hipSetDevice(cur);
+}
 
 dlu_status to_status(hipError_t e) {
   if (e == hipSuccess) return DLU_SUCCESS;
@@ -681,6 +768,11 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   if (!valid_arguments(m, n, d_a, lda, ipiv, jpiv)) return DLU_INVALID_ARGUMENT;
   int nb = (opt && opt->nb) ? opt->nb : 128;
   if (nb != 64 && nb != 128 && nb != 256) return DLU_INVALID_ARGUMENT;
+  const int variant = opt ? opt->fallback : DLU_FALLBACK_FULL;
+  if (variant != DLU_FALLBACK_FULL && variant != DLU_FALLBACK_COLUMN) return DLU_INVALID_ARGUMENT;
+  const int forced_rpl = opt ? opt->rows_per_lane : 0;
+  if (forced_rpl != 0 && forced_rpl != 1 && forced_rpl != 2 && forced_rpl != 4)
+    return DLU_INVALID_ARGUMENT;
   const bool lookahead = !(opt && opt->lookahead < 0);
   const long long spin =
       (opt && opt->spin_limit_ticks > 0) ? opt->spin_limit_ticks : 200000000ll;
@@ -703,14 +795,6 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
     *info = -1;
     return DLU_RUNTIME_FAILURE;
   }
-  int forced_rpl = 0;
-  if (opt && opt->rows_per_lane) {
-    if (opt->rows_per_lane != 1 && opt->rows_per_lane != 2 && opt->rows_per_lane != 4) {
-      *info = -1;
-      return DLU_INVALID_ARGUMENT;
-    }
-    forced_rpl = opt->rows_per_lane;
-  }
   int cus = 0;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) !=
           hipSuccess ||
@@ -724,14 +808,21 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   // panel workgroups (more than 4096 rows), enough CUs for one resident workgroup per panel row
   // block (at most half the device), which also lets the far GEMM's tail run on them behind each
   // panel; smaller factorizations keep the whole device, so several of them (serving) overlap.
-  // opt->panel_cus > 0 forces a size, < 0 disables.
+  // opt->panel_cus > 0 forces a size (rounded down to whole XCDs), < 0 disables.
+  // A partition is made of whole XCDs: a stream's CU mask fills the XCDs in order (32 mask bits
+  // each on MI355X), and a launch's workgroups are dealt round robin over the XCDs its mask
+  // touches, so a partly masked XCD would receive as many panel workgroups as a full one and
+  // could not hold them all at once (measured: 40 panel workgroups on a 48-CU mask never became
+  // co-resident; on 32 and 64 CUs they do).
   int groups = 0, rpl = 0;
   panel_shape_for(m, nb, forced_rpl, std::min(kMaxPanelGroups, cus), groups, rpl);
+  const int xcd = cus % kCusPerXcd == 0 ? kCusPerXcd : cus;  // unknown layout: no partition
   int want = 0;
   if (sizeof(T) == 4 && groups > 16)
-    want = std::max(16, std::min(((groups + 15) / 16) * 16, cus / 2));
-  if (opt && opt->panel_cus > 0) want = std::min(opt->panel_cus, cus);
+    want = std::min((groups + xcd - 1) / xcd * xcd, cus / 2 / xcd * xcd);
+  if (opt && opt->panel_cus > 0) want = std::min(opt->panel_cus, cus) / xcd * xcd;
   if (opt && opt->panel_cus < 0) want = 0;
+  if (want >= cus) want = 0;                // no CUs left for the updates: share the device
   if (groups > want && want > 0) want = 0;  // too tall for a partition: share the device
   Lease lease(device, std::min(groups, cus), want > 0, cus);
   Workspace &w = lease.workspace();
@@ -742,6 +833,7 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   Run<T> run{w, m, n, lda, steps, nb, d_a, stream, spin};
   run.profile = opt && opt->profile;
   run.forced_rpl = forced_rpl;
+  run.variant = variant;
   if (opt && opt->merge_rows != 0) run.merge_rows = opt->merge_rows < 0 ? (1 << 30) : opt->merge_rows;
   run.split = !(opt && opt->split_update < 0);
   if (opt && opt->split_update > 0) run.forced_split = opt->split_update;
@@ -762,15 +854,9 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
     *info = -1;
     return to_status(e);
   }
-  {
-    int probe_groups = 0, probe_rpl = 0;
-    if (!run.panel_shape(0, probe_groups, probe_rpl)) {
-      *info = -1;
-      return DLU_INVALID_ARGUMENT;  // more rows than the panel kernel can hold
-    }
-  }
   Control init{};
   init.min_margin = 2.0;
+  init.col_found = -1;
   *w.h_ctl = init;
   // Swap lists start empty with their append counters at zero (panels clear them after use;
   // this also recovers from an aborted call).
@@ -832,7 +918,7 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
 }
 
 dlu_status getrf_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv, int *info,
-                      hipStream_t stream) {
+                      hipStream_t stream, int variant) {
   if (info == nullptr) return DLU_INVALID_ARGUMENT;
   *info = -1;
   if (!valid_arguments(m, n, a, lda, ipiv, jpiv)) return DLU_INVALID_ARGUMENT;
@@ -847,7 +933,9 @@ dlu_status getrf_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv, in
   std::unique_ptr<double, hipError_t (*)(void *)> hold(d_a, hipFree);
   if (hipMemcpyAsync(d_a, a, bytes, hipMemcpyHostToDevice, stream) != hipSuccess)
     return DLU_RUNTIME_FAILURE;
-  const dlu_status st = getrf_device(m, n, d_a, lda, ipiv, jpiv, info, stream, nullptr, nullptr,
+  dlu_options opt{};
+  opt.fallback = variant;
+  const dlu_status st = getrf_device(m, n, d_a, lda, ipiv, jpiv, info, stream, &opt, nullptr,
                                      nullptr);
   if (st != DLU_SUCCESS) return st;
   if (hipMemcpyAsync(a, d_a, bytes, hipMemcpyDeviceToHost, stream) != hipSuccess ||
@@ -896,7 +984,8 @@ dlu_status dlu_sgetrf_device_ex(int m, int n, float *d_a, int lda, int *ipiv, in
 
 dlu_status dlu_getrf_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv, int *info,
                           void *stream) {
-  return dlu::getrf_host(m, n, a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream));
+  return dlu::getrf_host(m, n, a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream),
+                         DLU_FALLBACK_FULL);
 }
 
 dlu_status dlu_getrf_native_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
@@ -917,6 +1006,25 @@ dlu_status dlu_getrf_hybrid_device(int m, int n, double *d_a, int lda, int *ipiv
 dlu_status dlu_getrf_hybrid_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
                                  int *info, void *stream) {
   return dlu_getrf_host(m, n, a, lda, ipiv, jpiv, info, stream);
+}
+
+dlu_status dlu_getrf_column_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
+                                   int *info, void *stream) {
+  dlu_options opt{};
+  opt.fallback = DLU_FALLBACK_COLUMN;
+  return dlu::getrf_device(m, n, d_a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream),
+                           &opt, nullptr, nullptr);
+}
+
+dlu_status dlu_getrf_column_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                                 int *info, void *stream) {
+  return dlu::getrf_host(m, n, a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream),
+                         DLU_FALLBACK_COLUMN);
+}
+
+void dlu_shutdown(void) {
+  dlu::release_pools();
+  dlu::release_thread_solve_cache();
 }
 
 const char *dlu_status_string(dlu_status status) {

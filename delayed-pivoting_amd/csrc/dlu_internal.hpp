@@ -30,6 +30,7 @@ struct Control {
   int best_col;
   unsigned search_ticket;
   unsigned abort_flag;  // set by a panel workgroup whose wait expired; the others leave too
+  int col_found;        // column-variant search: rightmost nonzero column so far (-1: none)
 };
 
 // A panel's control block as the host sees it: workgroup 0 of the panel kernel copies the
@@ -39,6 +40,16 @@ struct PanelDone {
   Control c;
   unsigned long long seq;
 };
+
+// The panel's last word to the host: the control block, copied into pinned host memory, then
+// the sequence number with system-scope release (the host spins on it).
+__device__ __forceinline__ void publish_done(const Control *ctl, PanelDone *done,
+                                             unsigned long long seq) {
+  if (done == nullptr) return;
+  done->c = *ctl;
+  __threadfence_system();
+  __hip_atomic_store(&done->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // Exchange buffer geometry of the panel kernel (in 64-bit words).  Every word carries the step
 // tag of the step that wrote it, so a reader can tell a fresh word from a stale one without any
@@ -139,6 +150,28 @@ template <typename T>
 hipError_t launch_active_search(const T *a, int lda, int m, int n, int k, double *partial_v,
                                 unsigned long long *partial_o, Control *ctl, hipStream_t s);
 
+// Column-variant fallback search (delayed_getrf_column): the rightmost column j in (k, n) with a
+// nonzero entry in rows [k, m), and its largest |a| (lowest row on ties); result in ctl->best_*
+// (best_value = 0 when every such column is zero).
+template <typename T>
+hipError_t launch_column_search(const T *a, int lda, int m, int n, int k, Control *ctl,
+                                hipStream_t s);
+
+// Tall panels (more rows than the register-resident panel kernel holds): one launch sequence
+// per column step, every launch predicated on the control block.  begin: stop = pe (pb after a
+// deferral); per step k: the column argmax (launch_active_search over one column), then
+// tall_pivot (zero column: stop = k; else ipiv[k] and the whole-row interchange), then
+// tall_update (multipliers and the rank-1 update of the panel's columns); finish: the panel's
+// swap list (identity: rows already moved) and the host copy of the control block.
+hipError_t launch_tall_begin(Control *ctl, int pb, int pe, hipStream_t s);
+template <typename T>
+hipError_t launch_tall_pivot(T *a, int lda, int n, int k, int *ipiv, Control *ctl, hipStream_t s);
+template <typename T>
+hipError_t launch_tall_update(T *a, int lda, int m, int k, int pe, const Control *ctl,
+                              hipStream_t s);
+hipError_t launch_tall_finish(Control *ctl, int *swap_list, int nb, int pb, PanelDone *done,
+                              unsigned long long seq, hipStream_t s);
+
 // Apply the full-pivot decision at step k (row swap, then column swap) or mark the deferral.
 template <typename T>
 hipError_t launch_fallback_apply(T *a, int lda, int m, int n, int k, int *ipiv, int *jpiv,
@@ -217,6 +250,12 @@ dlu_status solve_refine(const DeviceFactor &f, int nrhs, const double *d_a, int 
                         int max_iter, hipStream_t s, int *iterations, double *bwd,
                         bool *converged);
 dlu_status to_status(hipError_t e);
+// Process teardown before the HIP runtime goes: the idle factorization workspaces of every
+// device, and the calling thread's solve-side scratch (each serving worker frees its own).
+void release_pools();
+void release_thread_solve_cache();
+// Device bytes held by the calling thread's solve-side scratch on `device`.
+size_t thread_solve_cache_bytes(int device);
 template <typename T>
 dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int *info,
                         hipStream_t stream, const dlu_options *opt, dlu_report *rep,

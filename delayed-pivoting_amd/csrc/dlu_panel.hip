@@ -256,16 +256,6 @@ __device__ __forceinline__ void as_word(float &x, int i) { x = __int_as_float(i)
 __device__ __forceinline__ int from_word(double x) { return static_cast<int>(lo32(x)); }
 __device__ __forceinline__ int from_word(float x) { return __float_as_int(x); }
 
-// The panel's last word to the host: the control block, copied into pinned host memory, then
-// the sequence number with system-scope release (the host spins on it).
-__device__ __forceinline__ void publish_done(const Control *ctl, PanelDone *done,
-                                             unsigned long long seq) {
-  if (done == nullptr) return;
-  done->c = *ctl;
-  __threadfence_system();
-  __hip_atomic_store(&done->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

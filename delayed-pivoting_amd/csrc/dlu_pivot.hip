@@ -131,6 +131,166 @@ __global__ __launch_bounds__(256) void active_search_kernel(const T *__restrict_
   }
 }
 
+// Column-variant fallback search at step k (delayed_lu/delayed_dgetrf_column.cpp:30-45): the
+// reference walks the columns right to left and stops at the first whose idamax over rows
+// [k, m) is nonzero.  Here workgroup b of G examines columns n-1-b, n-1-b-G, ... (right to left)
+// and stops at the first column that lies left of a nonzero column already found; a workgroup
+// stops reading a column at its first nonzero entry (8 coalesced loads in flight per lane).  The
+// rightmost nonzero column is kept with an atomic max, so the answer does not depend on
+// dispatch order; G stays small (at most 256) so the closing ticket costs few atomics.  The
+// last workgroup to finish then takes that column's largest |a| (lowest row on ties, as idamax)
+// and clears the search state.  NaN compares false: a column of NaN and zeros counts as zero.
+template <typename T>
+__global__ __launch_bounds__(256) void column_search_kernel(const T *__restrict__ a, int lda,
+                                                           int m, int n, int k, Control *ctl) {
+  constexpr int U = 8;
+  const int rows = m - k;
+  for (int j = n - 1 - static_cast<int>(blockIdx.x); j > k; j -= static_cast<int>(gridDim.x)) {
+    // Thread 0 reads the column found so far; the barrier's reduction makes the decision the
+    // same for every thread of the workgroup (all of them meet the barriers below).
+    const bool beaten =
+        threadIdx.x == 0 &&
+        __hip_atomic_load(&ctl->col_found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= j;
+    if (__syncthreads_or(beaten)) break;
+    const T *cp = a + static_cast<size_t>(k) + static_cast<size_t>(j) * lda;
+    bool hit = false;
+    for (int r0 = 0; r0 < rows; r0 += 256 * U) {
+      T x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int r = r0 + 256 * u + static_cast<int>(threadIdx.x);
+        x[u] = cp[r < rows ? r : 0];
+      }
+      bool nz = false;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int r = r0 + 256 * u + static_cast<int>(threadIdx.x);
+        nz = nz || (r < rows && fabs(static_cast<double>(x[u])) > 0.0);
+      }
+      if (__syncthreads_or(nz)) {
+        if (threadIdx.x == 0) atomicMax(&ctl->col_found, j);
+        hit = true;
+        break;
+      }
+    }
+    if (hit) break;  // columns further left cannot win
+  }
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(&ctl->search_ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const int c = __hip_atomic_load(&ctl->col_found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  double best = 0.0;
+  unsigned long long bo = 0ull;
+  if (c > k) {
+    const T *cp = a + static_cast<size_t>(k) + static_cast<size_t>(c) * lda;
+    for (int r0 = 0; r0 < rows; r0 += 256 * U) {
+      T x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int r = r0 + 256 * u + static_cast<int>(threadIdx.x);
+        x[u] = cp[r < rows ? r : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {  // a thread meets its rows in increasing order
+        const int r = r0 + 256 * u + static_cast<int>(threadIdx.x);
+        const double v = r < rows ? fabs(static_cast<double>(x[u])) : 0.0;
+        const bool take = v > best;
+        bo = take ? static_cast<unsigned long long>(r) : bo;
+        best = take ? v : best;
+      }
+    }
+  }
+  block_best(best, bo);
+  if (threadIdx.x == 0) {
+    ctl->best_value = best;
+    ctl->best_row = k + static_cast<int>(bo);
+    ctl->best_col = c > k ? c : k;
+    ctl->search_ticket = 0u;
+    ctl->col_found = -1;
+  }
+}
+
+// ---- Tall panels ---------------------------------------------------------------------------
+// A panel with more rows than the register-resident panel kernel can hold is factored one
+// column step at a time: the column argmax (active_search_kernel over the single column k),
+// tall_pivot_kernel, tall_update_kernel.  Every launch is predicated on ctl->stop, so a zero
+// column ends the panel on the device (stop = k) and the remaining launches return at once.
+// Rows are interchanged across all n columns as the reference does (delayed_dgetrf.cpp:202-205),
+// so the panel's swap list is the identity.
+
+__global__ void tall_begin_kernel(Control *ctl, int pb, int pe) {
+  if (threadIdx.x == 0) {
+    ctl->stop = ctl->deferred ? pb : pe;  // a speculative restart after a deferral does nothing
+    ctl->zero_column = 0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void tall_pivot_kernel(T *a, int lda, int n, int k, int *ipiv,
+                                                        Control *ctl) {
+  if (k >= ctl->stop) return;
+  const double best = ctl->best_value;
+  if (!(best > 0.0)) {  // zero (or NaN-only) column: the panel stops at k
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ctl->stop = k;
+      ctl->zero_column = 1;
+    }
+    return;
+  }
+  const int r = ctl->best_row;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[k] = r;
+  if (r == k) return;
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+    T *pk = a + static_cast<size_t>(k) + static_cast<size_t>(j) * lda;
+    T *pr = a + static_cast<size_t>(r) + static_cast<size_t>(j) * lda;
+    const T t = *pk;
+    *pk = *pr;
+    *pr = t;
+  }
+}
+
+// Multipliers of column k (times the reciprocal of the pivot, delayed_dgetrf.cpp:221-228) and the
+// rank-1 update of the panel's columns (k, pe) (:230-239), one row per thread.
+template <typename T>
+__global__ __launch_bounds__(256) void tall_update_kernel(T *a, int lda, int m, int k, int pe,
+                                                         const Control *ctl) {
+  if (k >= ctl->stop) return;
+  const int i = k + 1 + static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+  if (i >= m) return;
+  const T *rowk = a + static_cast<size_t>(k);
+  T *rowi = a + static_cast<size_t>(i);
+  const T recip = T(1) / rowk[static_cast<size_t>(k) * lda];
+  const T l = rowi[static_cast<size_t>(k) * lda] * recip;
+  rowi[static_cast<size_t>(k) * lda] = l;
+  for (int j = k + 1; j < pe; ++j)
+    rowi[static_cast<size_t>(j) * lda] =
+        fma(-l, rowk[static_cast<size_t>(j) * lda], rowi[static_cast<size_t>(j) * lda]);
+}
+
+// The panel's swap list: rows [pb, stop) in place (their interchanges are done), nothing
+// displaced; then the host's copy of the control block.
+__global__ void tall_finish_kernel(Control *ctl, int *swap_list, int nb, int pb, PanelDone *done,
+                                   unsigned long long seq) {
+  const int steps = ctl->stop - pb;
+  for (int i = threadIdx.x; i < steps; i += blockDim.x) {
+    swap_list[i] = pb + i;
+    swap_list[2 * nb + i] = pb + i;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    swap_list[4 * nb] = steps;
+    swap_list[4 * nb + 1] = 0;
+    swap_list[4 * nb + 2] = 0;
+    __threadfence();
+    publish_done(ctl, done, seq);
+  }
+}
+
 // Step k of a full-pivot fallback in one launch: record the decision, swap rows k and best_row
 // over all n columns, then columns k and best_col over all m rows.  The two swaps meet only in
 // the four elements of rows {k, r} x columns {k, c}; one thread does those, rows first, as the
@@ -235,6 +395,45 @@ hipError_t launch_active_search(const T *a, int lda, int m, int n, int k, double
 }
 
 template <typename T>
+hipError_t launch_column_search(const T *a, int lda, int m, int n, int k, Control *ctl,
+                                hipStream_t s) {
+  const int cols = n - k - 1;
+  const int grid = cols < 1 ? 1 : (cols > 256 ? 256 : cols);
+  hipLaunchKernelGGL(column_search_kernel<T>, dim3(grid), dim3(256), 0, s, a, lda, m, n, k, ctl);
+  return hipGetLastError();
+}
+
+hipError_t launch_tall_begin(Control *ctl, int pb, int pe, hipStream_t s) {
+  hipLaunchKernelGGL(tall_begin_kernel, dim3(1), dim3(64), 0, s, ctl, pb, pe);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_tall_pivot(T *a, int lda, int n, int k, int *ipiv, Control *ctl, hipStream_t s) {
+  const int g = (n + 255) / 256 < 512 ? (n + 255) / 256 : 512;
+  hipLaunchKernelGGL(tall_pivot_kernel<T>, dim3(g > 0 ? g : 1), dim3(256), 0, s, a, lda, n, k,
+                     ipiv, ctl);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_tall_update(T *a, int lda, int m, int k, int pe, const Control *ctl,
+                              hipStream_t s) {
+  const int rows = m - k - 1;
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(tall_update_kernel<T>, dim3((rows + 255) / 256), dim3(256), 0, s, a, lda, m,
+                     k, pe, ctl);
+  return hipGetLastError();
+}
+
+hipError_t launch_tall_finish(Control *ctl, int *swap_list, int nb, int pb, PanelDone *done,
+                              unsigned long long seq, hipStream_t s) {
+  hipLaunchKernelGGL(tall_finish_kernel, dim3(1), dim3(256), 0, s, ctl, swap_list, nb, pb, done,
+                     seq);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_fallback_apply(T *a, int lda, int m, int n, int k, int *ipiv, int *jpiv,
                                  Control *ctl, hipStream_t s) {
   const int len = m > n ? m : n;
@@ -263,7 +462,11 @@ hipError_t launch_row_interchange(T *a, int lda, int c0, int c1, const int *swap
                                               unsigned long long *, Control *, hipStream_t);    \
   template hipError_t launch_fallback_apply<T>(T *, int, int, int, int, int *, int *, Control *, \
                                                hipStream_t);                                     \
-  template hipError_t launch_row_interchange<T>(T *, int, int, int, const int *, int, hipStream_t);
+  template hipError_t launch_row_interchange<T>(T *, int, int, int, const int *, int, hipStream_t); \
+  template hipError_t launch_column_search<T>(const T *, int, int, int, int, Control *,          \
+                                              hipStream_t);                                     \
+  template hipError_t launch_tall_pivot<T>(T *, int, int, int, int *, Control *, hipStream_t);   \
+  template hipError_t launch_tall_update<T>(T *, int, int, int, int, const Control *, hipStream_t);
 DLU_PIVOT_INSTANTIATE(float)
 DLU_PIVOT_INSTANTIATE(double)
 #undef DLU_PIVOT_INSTANTIATE

@@ -32,6 +32,8 @@ using Clock = std::chrono::steady_clock;
 
 size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
+constexpr int kTuneBatch = 128;  // matrices per batched launch timed by the cutoff tuner
+
 struct Resident {
   long long id = -1;
   int n = 0;
@@ -66,6 +68,7 @@ struct Server {
   std::vector<dlu_completion> done;
   std::unordered_map<long long, Resident *> store;
   size_t used = 0;
+  size_t held = 0;  // the part of `used` reserved by requests in progress (freed when they end)
   unsigned long long tick = 0;
   long long inflight = 0;
   bool stop = false;
@@ -92,6 +95,9 @@ struct Server {
     return best;
   }
   // Returns false when the request can never fit, or (block = false) when it does not fit now.
+  // Waiting helps only while requests in progress hold memory they will give back: when none
+  // does, what fills the budget is pinned factors and the workers' scratch, and the request is
+  // admitted over the budget rather than waiting for memory that no one will free.
   bool reserve(size_t bytes, bool block) {
     std::unique_lock<std::mutex> lk(mu);
     if (bytes > budget) return false;
@@ -100,15 +106,18 @@ struct Server {
         evict_locked(v);
         continue;
       }
+      if (held == 0) break;
       if (!block || stop) return false;
       cv_mem.wait(lk);
     }
     used += bytes;
+    held += bytes;
     return true;
   }
   void release(size_t bytes) {
     std::lock_guard<std::mutex> lk(mu);
     used -= bytes;
+    held -= bytes;
     cv_mem.notify_all();
   }
   Resident *pin(long long id, bool batched) {
@@ -125,7 +134,9 @@ struct Server {
     r->stream = s;
     cv_mem.notify_all();
   }
+  // A finished request's factor memory becomes a resident factor (evictable once unpinned).
   void keep_locked(Resident *r) {
+    held -= r->bytes;
     auto it = store.find(r->id);
     if (it != store.end()) {  // an older factor under the same id: replace it when idle
       if (it->second->pins == 0) {
@@ -277,14 +288,39 @@ struct Server {
   }
 
   // ---- the full-factorization worker -------------------------------------------------------
+  // The worker's own solve-side scratch (it grows with the largest request served) counts
+  // against the HBM budget like the factors do: after each request the growth is charged,
+  // evicting idle resident factors if that overruns the budget.
+  void charge_cache(size_t &charged) {
+    const size_t now_bytes = thread_solve_cache_bytes(device);
+    if (now_bytes <= charged) return;
+    std::lock_guard<std::mutex> lk(mu);
+    used += now_bytes - charged;
+    charged = now_bytes;
+    while (used > budget)
+      if (Resident *v = lru_locked()) {
+        evict_locked(v);
+      } else {
+        break;
+      }
+  }
+
   void large_loop(hipStream_t s_large) {
     hipSetDevice(device);
+    size_t charged = 0;
     for (;;) {
       Job j;
       {
         std::unique_lock<std::mutex> lk(mu);
         cv_large.wait(lk, [&] { return stop || !large_q.empty(); });
-        if (stop && large_q.empty()) return;
+        if (stop && large_q.empty()) {
+          lk.unlock();
+          release_thread_solve_cache();  // this thread's scratch goes with it
+          lk.lock();
+          used -= charged;
+          cv_mem.notify_all();
+          return;
+        }
         j = large_q.front();
         large_q.pop_front();
       }
@@ -366,6 +402,7 @@ struct Server {
         hipFreeAsync(mem, s_large);
         release(bytes);
       }
+      charge_cache(charged);
       std::lock_guard<std::mutex> lk(mu);
       if (res) {
         res->pins--;
@@ -381,7 +418,7 @@ struct Server {
   // Per-matrix cost of the batched kernel (a round of 128 matrices, amortized) against one full
   // factorization + solve, at n = 32..128: the cutoff is the largest n where batching is cheaper.
   int tune_cutoff() {
-    const int B = 128;
+    const int B = kTuneBatch;  // h_items / d_items hold at least this many items
     int best = 0;
     double *a = nullptr, *b = nullptr, *lu = nullptr;
     int *piv = nullptr;
@@ -461,10 +498,12 @@ dlu_server *dlu_server_create(const dlu_server_config *config) {
   if (!(s->cfg.tol > 0.0)) s->cfg.tol = 1e-13;
   if (s->cfg.max_batch <= 0) s->cfg.max_batch = 1024;
   s->budget = static_cast<size_t>(s->cfg.hbm_budget_gib * 1073741824.0);
+  // Item arrays: a round's batch, and the cutoff tuner's (which may be the larger).
+  const size_t items = static_cast<size_t>(std::max(s->cfg.max_batch, dlu::kTuneBatch));
   if (hipGetDevice(&s->device) != hipSuccess ||
       hipStreamCreateWithFlags(&s->s_small, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc(&s->h_items, sizeof(dlu::BatchItem) * s->cfg.max_batch) != hipSuccess ||
-      hipMalloc(&s->d_items, sizeof(dlu::BatchItem) * s->cfg.max_batch) != hipSuccess ||
+      hipHostMalloc(&s->h_items, sizeof(dlu::BatchItem) * items) != hipSuccess ||
+      hipMalloc(&s->d_items, sizeof(dlu::BatchItem) * items) != hipSuccess ||
       hipHostMalloc(&s->h_infos, sizeof(int) * s->cfg.max_batch) != hipSuccess ||
       hipMalloc(&s->d_infos, sizeof(int) * s->cfg.max_batch) != hipSuccess) {
     delete s;

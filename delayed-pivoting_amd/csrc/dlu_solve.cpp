@@ -82,7 +82,9 @@ struct SolveCache {
     return e;
   }
 };
-// Per host thread (the serving workers and callers each keep their own scratch).
+// Per host thread (the serving workers and callers each keep their own scratch).  No
+// destructor frees it: a thread-local destructor may run after the HIP runtime is gone.  The
+// serving workers free theirs before they exit, callers through dlu_shutdown.
 thread_local SolveCache t_cache[64];
 
 // Net row order of P b (swaps in step order) and net order of Q y (swaps in reverse order):
@@ -390,6 +392,30 @@ dlu_status gesv(int n, int nrhs, const double *d_a, int lda, const double *d_b, 
 }
 
 }  // namespace
+
+void release_thread_solve_cache() {
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) return;
+  for (int d = 0; d < 64; ++d) {
+    SolveCache &c = t_cache[d];
+    bool any = false;
+    for (void *p : c.buf) any = any || p != nullptr;
+    if (!any || hipSetDevice(d) != hipSuccess) continue;
+    hipDeviceSynchronize();  // a solve of this thread may still be queued
+    for (void *p : c.buf)
+      if (p) hipFree(p);
+    c = SolveCache{};
+  }
+  hipSetDevice(cur);
+}
+
+size_t thread_solve_cache_bytes(int device) {
+  if (device < 0 || device >= 64) return 0;
+  size_t total = 0;
+  for (size_t b : t_cache[device].cap) total += b;
+  return total;
+}
+
 }  // namespace dlu
 
 extern "C" {

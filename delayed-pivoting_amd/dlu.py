@@ -7,6 +7,7 @@ if the library is missing or a GPU call fails, these functions raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import enum
 import os
@@ -31,7 +32,12 @@ class Options(ctypes.Structure):
     _fields_ = [("nb", ctypes.c_int), ("lookahead", ctypes.c_int),
                 ("spin_limit_ticks", ctypes.c_longlong), ("profile", ctypes.c_int),
                 ("panel_cus", ctypes.c_int), ("rows_per_lane", ctypes.c_int),
-                ("merge_rows", ctypes.c_int), ("split_update", ctypes.c_int)]
+                ("merge_rows", ctypes.c_int), ("split_update", ctypes.c_int),
+                ("fallback", ctypes.c_int)]
+
+
+FALLBACK_FULL = 0    # delayed_getrf: largest |a| of the active block
+FALLBACK_COLUMN = 1  # delayed_getrf_column: rightmost nonzero column
 
 
 class Report(ctypes.Structure):
@@ -116,7 +122,8 @@ def lib():
     sig = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip,
            ctypes.c_void_p]
     for name in ("dlu_getrf_device", "dlu_getrf_host", "dlu_getrf_native_device",
-                 "dlu_getrf_native_host", "dlu_getrf_hybrid_device", "dlu_getrf_hybrid_host"):
+                 "dlu_getrf_native_host", "dlu_getrf_hybrid_device", "dlu_getrf_hybrid_host",
+                 "dlu_getrf_column_device", "dlu_getrf_column_host"):
         fn = getattr(L, name)
         fn.argtypes = sig
         fn.restype = ctypes.c_int
@@ -162,8 +169,19 @@ def lib():
     L.dlu_server_now.restype = ctypes.c_double
     L.dlu_server_destroy.argtypes = [ctypes.c_void_p]
     L.dlu_server_destroy.restype = None
+    L.dlu_shutdown.argtypes = []
+    L.dlu_shutdown.restype = None
     _lib = L
+    # Free libdlu's cached device state while the HIP runtime is still up (atexit handlers run
+    # before the interpreter and the runtime's own teardown).
+    atexit.register(shutdown)
     return L
+
+
+def shutdown() -> None:
+    """dlu_shutdown(): release libdlu's cached workspaces and this thread's solve scratch."""
+    if _lib is not None:
+        _lib.dlu_shutdown()
 
 
 def net_order(piv, steps: int, n: int) -> np.ndarray:
@@ -265,7 +283,7 @@ def getrf_device(m: int, n: int, d_a: int, lda: int, stream=None, options: Optio
         return (Status(st), ipiv[:p], jpiv[:p], info.value, rep.as_dict(),
                 marg[:p] if margins else None)
     fn = {"device": L.dlu_getrf_device, "native": L.dlu_getrf_native_device,
-          "hybrid": L.dlu_getrf_hybrid_device}[variant]
+          "hybrid": L.dlu_getrf_hybrid_device, "column": L.dlu_getrf_column_device}[variant]
     st = fn(m, n, d_a, lda, ip, jp, ctypes.byref(info), s)
     return Status(st), ipiv[:p], jpiv[:p], info.value, None, None
 
@@ -333,7 +351,7 @@ def getrf_host(a: np.ndarray, variant: str = "host", stream=None):
     jpiv = np.zeros(max(p, 1), dtype=np.int32)
     info = ctypes.c_int(-7)
     fn = {"host": L.dlu_getrf_host, "native": L.dlu_getrf_native_host,
-          "hybrid": L.dlu_getrf_hybrid_host}[variant]
+          "hybrid": L.dlu_getrf_hybrid_host, "column": L.dlu_getrf_column_host}[variant]
     st = fn(m, n, f.ctypes.data, max(1, m), ipiv.ctypes.data_as(_ip), jpiv.ctypes.data_as(_ip),
             ctypes.byref(info), _stream_ptr(stream))
     return Status(st), f, ipiv[:p], jpiv[:p], info.value

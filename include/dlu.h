@@ -64,6 +64,26 @@ dlu_status dlu_getrf_hybrid_host(int m, int n, double *a, int lda, int *ipiv, in
 /* Replaces delayed_getrf_cuda_status_string (delayed_dgetrf_cuda.hpp:63). */
 const char *dlu_status_string(dlu_status status);
 
+/* The column-delay variant on the GPU: delayed_getrf_column (delayed_lu/delayed_dgetrf_column.hpp:19,
+ * delayed_dgetrf_column.cpp:99-211), whose reference exists on the CPU only.  Same arguments,
+ * storage and rules as dlu_getrf_device, except the fallback taken when the active part of
+ * column k is zero: instead of the largest |a| of the whole active block, the rightmost column
+ * j > k with a nonzero entry in rows [k, m) is brought to position k (its largest |a|, lowest row
+ * on ties, becomes the pivot), so zero columns collect at the right end
+ * (delayed_dgetrf_column.cpp:30-45).  The _host form copies lda*n doubles in and out, as
+ * dlu_getrf_host does. */
+dlu_status dlu_getrf_column_device(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
+                                   int *info, void *stream);
+dlu_status dlu_getrf_column_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                                 int *info, void *stream);
+
+/* No counterpart in the reference (whose calls free everything before returning): frees the
+ * device memory, pinned host memory, streams and events libdlu keeps between calls for the
+ * calling process's idle factorizations and the calling thread's solves.  Call it before the
+ * process exits (the Python binding registers it with atexit) so that nothing of libdlu is left
+ * for the HIP runtime to tear down; later calls allocate again as needed. */
+void dlu_shutdown(void);
+
 /* ---- Extensions with no counterpart in the reference ------------------------------------ */
 
 /* Tuning knobs.  Zero-initialised fields take the defaults. */
@@ -82,7 +102,13 @@ typedef struct dlu_options {
                         next panel is done, sized so both streams finish together: 0 = automatic
                         (only when panel_cus reserves CUs), < 0 = never, > 0 = always that many
                         columns */
+  int fallback;      /* what a zero column falls back to: DLU_FALLBACK_FULL (delayed_getrf: the
+                        largest |a| of the active block) or DLU_FALLBACK_COLUMN
+                        (delayed_getrf_column: the rightmost nonzero column) */
 } dlu_options;
+
+#define DLU_FALLBACK_FULL 0
+#define DLU_FALLBACK_COLUMN 1
 
 /* What happened during one factorization. */
 typedef struct dlu_report {
@@ -234,6 +260,8 @@ typedef struct dlu_server_config {
                             (default 3, at most 4) */
 } dlu_server_config;
 
+/* The service's streams are not ordered after the caller's: a, b and x must hold their final
+ * contents (or allocations) before dlu_server_submit, e.g. after a stream synchronization. */
 typedef struct dlu_request {
   long long id;        /* caller's id, unique among live requests */
   int n, nrhs;         /* nrhs <= DLU_BATCH_MAX_RHS for batched requests */

@@ -7,6 +7,7 @@
  *   delayed_lu/delayed_dgetrf.cpp:32-47    (full-pivot search: idamax per column, strict '>')
  *   delayed_lu/delayed_dgetrf.cpp:61-85    (finish_panel: dtrsm + dgemm)
  *   delayed_lu/delayed_dgetrf.cpp:101-125  (materialize the accepted panel prefix)
+ *   delayed_lu/delayed_dgetrf_column.cpp:30-45, 99-211 (the column-delay variant's fallback)
  *   delayed_lu/lapack_dgetrf.cpp:21-63     (0-based LAPACK wrapper, info mapping)
  *   delayed_lu/example.cpp:43-126          (reconstruction of P*A*Q from the compact output)
  * The BLAS call sequence is kept the same as the reference's because the oracle's job is to
@@ -42,6 +43,7 @@ typedef void (*fn_dtrsm)(int, int, int, int, int, int, int, double, const double
 typedef void (*fn_dgemm)(int, int, int, int, int, int, double, const double *, int,
                          const double *, int, double, double *, int);
 typedef void (*fn_dgetrf)(const int *, const int *, double *, const int *, int *, int *);
+typedef void (*fn_sgetrf)(const int *, const int *, float *, const int *, int *, int *);
 typedef void (*fn_set_threads)(int);
 
 static struct {
@@ -53,6 +55,7 @@ static struct {
   fn_dtrsm dtrsm;
   fn_dgemm dgemm;
   fn_dgetrf dgetrf;
+  fn_sgetrf sgetrf;
   fn_set_threads set_threads;
   char name[256];
 } blas;
@@ -75,6 +78,7 @@ int dlu_oracle_init(const char *path, const char *prefix) {
   blas.dtrsm = (fn_dtrsm)sym(prefix, "cblas_dtrsm");
   blas.dgemm = (fn_dgemm)sym(prefix, "cblas_dgemm");
   blas.dgetrf = (fn_dgetrf)sym(prefix, "dgetrf_");
+  blas.sgetrf = (fn_sgetrf)sym(prefix, "sgetrf_"); /* optional: FP32 partial-pivoting pin */
   if (!blas.idamax || !blas.dswap || !blas.dscal || !blas.dger || !blas.dtrsm || !blas.dgemm ||
       !blas.dgetrf) {
     dlclose(h);
@@ -128,6 +132,35 @@ static double active_block_max(int m, int n, const double *a, int lda, int k, in
   return best;
 }
 
+/* Column-variant fallback (delayed_lu/delayed_dgetrf_column.cpp:30-45): walk the columns from
+ * the right end down to k+1 and stop at the first whose idamax over rows [k, m) is nonzero. */
+static double rightmost_nonzero_column(int m, int n, const double *a, int lda, int k, int *row,
+                                       int *col) {
+  *row = k;
+  *col = k;
+  for (int j = n - 1; j > k; --j) {
+    const int r = k + (int)blas.idamax(m - k, &AT(a, lda, k, j), 1);
+    const double v = fabs(AT(a, lda, r, j));
+    if (v > 0.0) {
+      *row = r;
+      *col = j;
+      return v;
+    }
+  }
+  return 0.0;
+}
+
+/* Column pivot of step k: idamax over A(k:m-1, k).  Built with DLU_ORACLE_MUTANT_TIES (the
+ * tests' planted mistake only), ties go to the highest row instead. */
+static int column_pivot_row(int m, const double *colk) {
+  int r = (int)blas.idamax(m, colk, 1);
+#ifdef DLU_ORACLE_MUTANT_TIES
+  for (int i = r + 1; i < m; ++i)
+    if (fabs(colk[i]) == fabs(colk[r])) r = i;
+#endif
+  return r;
+}
+
 /* U12 = L11^{-1} A12 and A22 -= L21 U12 for the panel [begin, end) (delayed_dgetrf.cpp:61-85). */
 static void close_panel(int m, int n, double *a, int lda, int begin, int end) {
   const int w = end - begin, right = n - end, below = m - end;
@@ -169,8 +202,10 @@ static double top2_margin(const double *x, int len) {
   return t1 > 0.0 ? (t1 - t2) / t1 : 0.0;
 }
 
-void dlu_oracle_delayed_getrf(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
-                              int *info, int nb, double *margins, int *fallbacks) {
+/* The blocked driver of both reference variants; `column` selects the fallback search
+ * (delayed_dgetrf_column.cpp:99-211 differs from delayed_dgetrf.cpp:129-251 only there). */
+static void delayed_lu(int m, int n, double *a, int lda, int *ipiv, int *jpiv, int *info, int nb,
+                       double *margins, int *fallbacks, int column) {
   /* Argument rules of delayed_dgetrf.cpp:132-151. */
   *info = -1;
   if (fallbacks) *fallbacks = 0;
@@ -196,7 +231,7 @@ void dlu_oracle_delayed_getrf(int m, int n, double *a, int lda, int *ipiv, int *
       } else {
         /* Column pivot first: idamax over A(k:m-1, k). */
         const double *colk = &AT(a, lda, k, k);
-        int prow = k + (int)blas.idamax(m - k, colk, 1);
+        int prow = k + column_pivot_row(m - k, colk);
         int pcol = k;
         const double alpha = fabs(AT(a, lda, prow, k));
         const int column_is_zero = !(alpha > 0.0); /* NaN reads as unacceptable too */
@@ -204,7 +239,8 @@ void dlu_oracle_delayed_getrf(int m, int n, double *a, int lda, int *ipiv, int *
         if (column_is_zero) {
           if (fallbacks) ++*fallbacks;
           if (k > panel_begin) apply_prefix_to_far_columns(m, n, a, lda, panel_begin, k, panel_end);
-          const double best = active_block_max(m, n, a, lda, k, &prow, &pcol);
+          const double best = column ? rightmost_nonzero_column(m, n, a, lda, k, &prow, &pcol)
+                                     : active_block_max(m, n, a, lda, k, &prow, &pcol);
           if (!(best > 0.0)) {
             *info = k; /* the whole active block is zero: [k, steps) is the delayed suffix */
             return;
@@ -238,6 +274,16 @@ void dlu_oracle_delayed_getrf(int m, int n, double *a, int lda, int *ipiv, int *
   }
 }
 
+void dlu_oracle_delayed_getrf(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                              int *info, int nb, double *margins, int *fallbacks) {
+  delayed_lu(m, n, a, lda, ipiv, jpiv, info, nb, margins, fallbacks, 0);
+}
+
+void dlu_oracle_delayed_getrf_column(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                                     int *info, int nb, double *margins, int *fallbacks) {
+  delayed_lu(m, n, a, lda, ipiv, jpiv, info, nb, margins, fallbacks, 1);
+}
+
 void dlu_oracle_lapack_getrf(int m, int n, double *a, int lda, int *ipiv, int *info) {
   *info = -1;
   if (m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return;
@@ -255,7 +301,35 @@ void dlu_oracle_lapack_getrf(int m, int n, double *a, int lda, int *ipiv, int *i
     *info = lapack_info - 1; /* first exactly-zero U diagonal, 0-based */
 }
 
+static void unblocked(int m, int n, double *a, int lda, int *ipiv, int *jpiv, int *info,
+                      int column);
+
+int dlu_oracle_lapack_sgetrf(int m, int n, float *a, int lda, int *ipiv, int *info) {
+  *info = -1;
+  if (blas.sgetrf == NULL) return 1;
+  if (m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return 0;
+  const int steps = m < n ? m : n;
+  int lapack_info = 0;
+  int *one_based = (int *)malloc(sizeof(int) * (size_t)(steps > 0 ? steps : 1));
+  if (one_based == NULL) return 2;
+  blas.sgetrf(&m, &n, a, &lda, one_based, &lapack_info);
+  for (int k = 0; k < steps; ++k) ipiv[k] = one_based[k] - 1;
+  free(one_based);
+  *info = lapack_info == 0 ? steps : (lapack_info > 0 ? lapack_info - 1 : -1);
+  return 0;
+}
+
 void dlu_oracle_unblocked_ld(int m, int n, double *a, int lda, int *ipiv, int *jpiv, int *info) {
+  unblocked(m, n, a, lda, ipiv, jpiv, info, 0);
+}
+
+void dlu_oracle_unblocked_column_ld(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                                    int *info) {
+  unblocked(m, n, a, lda, ipiv, jpiv, info, 1);
+}
+
+static void unblocked(int m, int n, double *a, int lda, int *ipiv, int *jpiv, int *info,
+                      int column) {
   *info = -1;
   if (m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return;
   const int steps = m < n ? m : n;
@@ -280,7 +354,18 @@ void dlu_oracle_unblocked_ld(int m, int n, double *a, int lda, int *ipiv, int *j
         pr = i;
       }
     }
-    if (!(best > 0.0L)) {
+    if (!(best > 0.0L) && column) {
+      /* rightmost column with a nonzero entry, its first largest |a| */
+      for (int j = n - 1; j > k && !(best > 0.0L); --j)
+        for (int i = k; i < m; ++i) {
+          const long double v = fabsl(W(i, j));
+          if (v > best) {
+            best = v;
+            pr = i;
+            pc = j;
+          }
+        }
+    } else if (!(best > 0.0L)) {
       for (int j = k; j < n; ++j)
         for (int i = k; i < m; ++i) {
           const long double v = fabsl(W(i, j));
@@ -290,10 +375,10 @@ void dlu_oracle_unblocked_ld(int m, int n, double *a, int lda, int *ipiv, int *j
             pc = j;
           }
         }
-      if (!(best > 0.0L)) {
-        *info = k;
-        break;
-      }
+    }
+    if (!(best > 0.0L)) { /* the whole active block is zero */
+      *info = k;
+      break;
     }
     ipiv[k] = pr;
     jpiv[k] = pc;

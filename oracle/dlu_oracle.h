@@ -10,6 +10,9 @@
  *     - full-pivot search      delayed_lu/delayed_dgetrf.cpp:32-47
  *     - finish_panel           delayed_lu/delayed_dgetrf.cpp:61-85
  *     - materialize prefix     delayed_lu/delayed_dgetrf.cpp:101-125
+ *   dlu_oracle_delayed_getrf_column
+ *                              delayed_lu/delayed_dgetrf_column.cpp:99-211 (the column-delay
+ *                              variant; fallback find_nonzero_column, :30-45)
  *   dlu_oracle_lapack_getrf    delayed_lu/lapack_dgetrf.cpp:21-63
  *   dlu_oracle_backward_error  delayed_lu/example.cpp:43-126 (reconstruction of P*A*Q from the
  *                              compact [L | U / Schur] storage), reported in Frobenius norm.
@@ -42,12 +45,26 @@ int dlu_oracle_set_threads(int threads);
 void dlu_oracle_delayed_getrf(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
                               int *info, int nb, double *margins, int *fallbacks);
 
+/* The column-delay variant, restating delayed_getrf_column
+ * (delayed_lu/delayed_dgetrf_column.cpp:99-211): a zero column falls back to the rightmost column
+ * j > k with a nonzero entry in rows [k, m) (find_nonzero_column, :30-45).  Same arguments. */
+void dlu_oracle_delayed_getrf_column(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                                     int *info, int nb, double *margins, int *fallbacks);
+
 /* LAPACK dgetrf_ with 0-based ipiv, restating lapack_getrf. */
 void dlu_oracle_lapack_getrf(int m, int n, double *a, int lda, int *ipiv, int *info);
+
+/* LAPACK sgetrf_ (FP32 partial pivoting) with 0-based ipiv: with no zero columns the FP32
+ * delayed LU is FP32 partial pivoting, so this pins its pivots up to near-ties.  Returns 1 when
+ * the loaded library has no sgetrf_. */
+int dlu_oracle_lapack_sgetrf(int m, int n, float *a, int lda, int *ipiv, int *info);
 
 /* Unblocked full restatement in long double on a copy (no BLAS).  `a` is overwritten by the
  * rounded-to-double factors. */
 void dlu_oracle_unblocked_ld(int m, int n, double *a, int lda, int *ipiv, int *jpiv, int *info);
+/* The same for the column-delay variant. */
+void dlu_oracle_unblocked_column_ld(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                                    int *info);
 
 /* ||P*A*Q - Lbar*Ubar||_F / (max(m,n) * eps * ||A||_F) for the compact output of any of the
  * routines above.  `jpiv` may be NULL (row pivoting only).  `eliminated` is info (the number

@@ -26,18 +26,37 @@ SOURCES = [os.path.join(HERE, "dlu_oracle.c")]
 _lib = None
 
 
-def build(force: bool = False) -> str:
-    """Compile the oracle with gcc (no BLAS at link time; it is opened at run time)."""
-    os.makedirs(BUILD_DIR, exist_ok=True)
+def build(force: bool = False, out: str = LIB_PATH, defines=()) -> str:
+    """Compile the oracle with gcc (no BLAS at link time; it is opened at run time).
+
+    ``defines`` (e.g. ``["DLU_ORACLE_MUTANT_TIES"]``) and ``out`` serve the tests' planted-mistake
+    build, which goes to a path of its own."""
+    os.makedirs(os.path.dirname(out), exist_ok=True)
     newest = max(os.path.getmtime(s) for s in SOURCES + [os.path.join(HERE, "dlu_oracle.h")])
-    if not force and os.path.exists(LIB_PATH) and os.path.getmtime(LIB_PATH) >= newest:
-        return LIB_PATH
-    tmp = LIB_PATH + ".tmp"
+    if not force and os.path.exists(out) and os.path.getmtime(out) >= newest:
+        return out
+    tmp = out + ".tmp"
     cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
-           "-o", tmp] + SOURCES + ["-ldl", "-lm"]
+           "-o", tmp] + [f"-D{d}" for d in defines] + SOURCES + ["-ldl", "-lm"]
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, out)
+    return out
+
+
+def load_build(path: str):
+    """Open another build of the oracle (same BLAS as lib()) for delayed_getrf(library=...)."""
+    lib()
+    L = ctypes.CDLL(path)
+    L.dlu_oracle_init.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    for name in ("dlu_oracle_delayed_getrf", "dlu_oracle_delayed_getrf_column"):
+        getattr(L, name).argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip, ip, ip,
+                                     ctypes.c_int, dp, ip]
+    for path_, prefix in _blas_candidates():
+        if L.dlu_oracle_init(path_.encode(), prefix.encode()) == 0:
+            return L
+    raise RuntimeError("oracle build: no usable BLAS/LAPACK found")
 
 
 def _blas_candidates():
@@ -75,7 +94,10 @@ def lib():
     ip = ctypes.POINTER(ctypes.c_int)
     L.dlu_oracle_delayed_getrf.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip, ip,
                                            ip, ctypes.c_int, dp, ip]
+    L.dlu_oracle_delayed_getrf_column.argtypes = L.dlu_oracle_delayed_getrf.argtypes
     L.dlu_oracle_lapack_getrf.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip, ip]
+    L.dlu_oracle_unblocked_column_ld.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int,
+                                                 ip, ip, ip]
     L.dlu_oracle_unblocked_ld.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip, ip,
                                           ip]
     L.dlu_oracle_backward_error.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int,
@@ -113,12 +135,16 @@ def _fortran(a: np.ndarray) -> np.ndarray:
     return np.array(a, dtype=np.float64, order="F", copy=True)
 
 
-def delayed_getrf(a: np.ndarray, nb: int = 64, margins: bool = False):
-    """Restated delayed_getrf on a copy of ``a`` (m x n).
+def delayed_getrf(a: np.ndarray, nb: int = 64, margins: bool = False, column: bool = False,
+                  library=None):
+    """Restated delayed_getrf (``column=True``: delayed_getrf_column) on a copy of ``a``.
 
     Returns ``(lu, ipiv, jpiv, info, extra)`` where ``extra`` holds ``fallbacks`` and, when
-    requested, the per-step column-pivot ``margins``.
+    requested, the per-step column-pivot ``margins``.  ``library`` may name another build of the
+    oracle (the tests' planted-mistake build); by default the one ``build()`` makes.
     """
+    L = library or lib()
+    fn = L.dlu_oracle_delayed_getrf_column if column else L.dlu_oracle_delayed_getrf
     f = _fortran(a)
     m, n = f.shape
     p = min(m, n)
@@ -127,8 +153,8 @@ def delayed_getrf(a: np.ndarray, nb: int = 64, margins: bool = False):
     info = ctypes.c_int(-1)
     fb = ctypes.c_int(0)
     marg = np.zeros(max(p, 1), dtype=np.float64) if margins else None
-    lib().dlu_oracle_delayed_getrf(m, n, _dp(f), max(1, m), _ip(ipiv), _ip(jpiv),
-                                   ctypes.byref(info), nb, _dp(marg), ctypes.byref(fb))
+    fn(m, n, _dp(f), max(1, m), _ip(ipiv), _ip(jpiv), ctypes.byref(info), nb, _dp(marg),
+       ctypes.byref(fb))
     extra = {"fallbacks": fb.value}
     if margins:
         extra["margins"] = marg[:p]
@@ -145,15 +171,35 @@ def lapack_getrf(a: np.ndarray):
     return f, ipiv[:p], info.value
 
 
-def unblocked_ld(a: np.ndarray):
+def lapack_sgetrf(a: np.ndarray):
+    """LAPACK sgetrf on a float32 copy: (lu32, ipiv 0-based, info)."""
+    L = lib()
+    f = np.array(a, dtype=np.float32, order="F", copy=True)
+    m, n = f.shape
+    p = min(m, n)
+    ipiv = np.zeros(max(p, 1), dtype=np.int32)
+    info = ctypes.c_int(-1)
+    L.dlu_oracle_lapack_sgetrf.argtypes = [ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_int)]
+    rc = L.dlu_oracle_lapack_sgetrf(m, n, f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                    max(1, m), _ip(ipiv), ctypes.byref(info))
+    if rc != 0:
+        raise RuntimeError("the loaded LAPACK has no sgetrf_")
+    return f, ipiv[:p], info.value
+
+
+def unblocked_ld(a: np.ndarray, column: bool = False):
+    """The independent unblocked long-double restatement (no BLAS), either variant."""
     f = _fortran(a)
     m, n = f.shape
     p = min(m, n)
     ipiv = np.zeros(max(p, 1), dtype=np.int32)
     jpiv = np.zeros(max(p, 1), dtype=np.int32)
     info = ctypes.c_int(-1)
-    lib().dlu_oracle_unblocked_ld(m, n, _dp(f), max(1, m), _ip(ipiv), _ip(jpiv),
-                                  ctypes.byref(info))
+    fn = lib().dlu_oracle_unblocked_column_ld if column else lib().dlu_oracle_unblocked_ld
+    fn(m, n, _dp(f), max(1, m), _ip(ipiv), _ip(jpiv), ctypes.byref(info))
     return f, ipiv[:p], jpiv[:p], info.value
 
 

@@ -58,13 +58,28 @@ def test_null_pointers():
     assert info.value == -1
 
 
-def test_lda_times_n_overflow():
+def test_invalid_options_rejected_before_device_work():
+    # an unsupported panel width, rows per lane or fallback variant is rejected (info = -1)
+    # before any device call, so this runs without a GPU
+    for opts in ({"nb": 96}, {"rows_per_lane": 3}, {"fallback": 2}, {"fallback": -1}):
+        st, ipiv, jpiv, inf, rep, _ = dlu.getrf_device(4, 4, 1234, 4, options=opts)
+        assert st == dlu.Status.INVALID_ARGUMENT and inf == -1, opts
+
+
+def test_column_variant_argument_rules():
+    # delayed_lu/delayed_dgetrf_column.cpp:101-112: the same argument rules as delayed_getrf
     ip = np.zeros(4, dtype=np.int32)
+    L = dlu.lib()
+    for m, n, a, lda in ((-1, 3, 1234, 3), (3, 3, 1234, 2), (3, 3, None, 3)):
+        info = ctypes.c_int(7)
+        for fn in (L.dlu_getrf_column_device, L.dlu_getrf_column_host):
+            assert fn(m, n, a, lda, ip.ctypes.data_as(dlu._ip), ip.ctypes.data_as(dlu._ip),
+                      ctypes.byref(info), None) == dlu.Status.INVALID_ARGUMENT
+            assert info.value == -1
     info = ctypes.c_int(7)
-    # lda*n*8 fits in size_t on 64-bit for any int pair, so this must be accepted as valid up to
-    # the point where a device is needed; instead check the nb option validation path.
-    st, ipiv, jpiv, inf, rep, _ = dlu.getrf_device(4, 4, 1234, 4, options={"nb": 96})
-    assert st == dlu.Status.INVALID_ARGUMENT and inf == -1
+    assert L.dlu_getrf_column_device(0, 5, None, 1, ip.ctypes.data_as(dlu._ip),
+                                     ip.ctypes.data_as(dlu._ip), ctypes.byref(info), None) == 0
+    assert info.value == 0
 
 
 @pytest.mark.parametrize("m,n", [(0, 5), (5, 0), (0, 0)])

@@ -332,3 +332,35 @@ def test_schedules_match_oracle(sched, case):
     check_parity(a, lu, ipiv, jpiv, info, marg)
     ref = gpu_factor(a, nb=128)
     assert np.array_equal(ipiv, ref[1]) and np.array_equal(jpiv, ref[2]) and info == ref[3]
+
+
+def test_stream_pairs_of_other_partitions_are_replaced():
+    """FP32 factorizations above 4096 rows reserve panel CUs sized by their row count, so four of
+    different sizes fill a workspace's four stream pairs with four partitions.  An FP64 call on
+    the same thread asks for no partition: it must get an unpartitioned pair (a replaced slot),
+    not a partitioned one whose panel stream is too small for its 64 panel workgroups."""
+    torch = _torch()
+    for n in (5000, 10000, 16384, 20000):
+        t = torch.randn((n, n), dtype=torch.float32, device="cuda")
+        st, ipiv, jpiv, info, rep, _ = dlu.sgetrf_device(n, n, t.data_ptr(), n)
+        torch.cuda.synchronize()
+        assert st == dlu.Status.SUCCESS and info == n, (n, dlu.status_string(st))
+        del t
+    n = 16384
+    t = torch.randn((n, n), dtype=torch.float64, device="cuda")
+    st, ipiv, jpiv, info, rep, _ = dlu.getrf_device(n, n, t.data_ptr(), n)
+    torch.cuda.synchronize()
+    assert st == dlu.Status.SUCCESS and info == n, dlu.status_string(st)
+    del t
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("panel_cus", [200, 240])
+def test_fallback_restart_with_forced_panel_cus(panel_cus):
+    """With the panel stream given most of the CUs, the update stream keeps too few for a panel's
+    workgroups: the panel restarted after a full-pivot fallback must run on the panel stream."""
+    a, zc, _, _ = workloads.delay_stress(4096, 17)
+    lu, ipiv, jpiv, info, rep, marg = gpu_factor(a, nb=128, sched={"panel_cus": panel_cus})
+    assert info == 4096 - len(zc)
+    assert rep["fallbacks"] >= len(zc)
+    check_parity(a, lu, ipiv, jpiv, info, marg)

@@ -230,3 +230,35 @@ def test_server_mixed_mode_reaches_fp64_backward_error():
     r = np.abs(b - a @ x).max(axis=0)
     den = np.abs(a).sum(axis=1).max() * np.abs(x).max(axis=0) + np.abs(b).max(axis=0)
     assert (r / den).max() <= 1e-13
+
+
+def test_server_small_max_batch_with_tuned_cutoff():
+    """max_batch below the cutoff tuner's batch (128 matrices) with the cutoff tuned at
+    creation: the tuner's item arrays must still fit, and the service must work."""
+    reqs = _requests(6, 30, [40, 64, 100, 300])
+    comps, xs, cut = _serve(reqs, False, batch_cutoff=0, max_batch=16)
+    assert cut >= 32
+    for r in reqs:
+        assert comps[r["id"]]["status"] == 0
+        ref = np.linalg.solve(r["a"], r["b"])
+        assert np.linalg.norm(xs[r["id"]] - ref) <= 1e-10 * np.linalg.norm(ref)
+
+
+def test_server_create_destroy_returns_device_memory():
+    """Five servers created, used and destroyed: the workers' solve scratch goes with them, so
+    free device memory returns to where it was after the first (which warms the per-device
+    factorization workspaces that outlive servers by design)."""
+    torch = _torch()
+    reqs = _requests(7, 12, [4096, 3000, 64, 2048])
+
+    def cycle():
+        _serve(reqs, False, batch_cutoff=128)
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        return torch.cuda.mem_get_info()[0]
+
+    cycle()
+    base = cycle()
+    free = [cycle() for _ in range(4)]
+    print("free bytes after each cycle:", base, free, flush=True)
+    assert min(free) >= base - (16 << 20)

@@ -252,3 +252,23 @@ def test_gesv_mixed_singular_input_falls_back_and_reports():
                                  mixed=True)
     assert st == dlu.Status.INVALID_ARGUMENT
     assert rep["factor_info"] == n - 1 and rep["used_fp64"] == 1
+
+
+@pytest.mark.parametrize("n,seed", [(6000, 21), (8000, 22)])
+def test_sgetrf_default_path_above_4096(n, seed):
+    """The FP32 defaults above 4096 rows: a CU-partitioned panel stream, the far GEMM's tail on
+    the panel stream, 128 x 128 FP32 GEMM tiles and the pair schedule.  Pivots equal LAPACK
+    sgetrf's up to a reported FP32 near-tie; the factor's backward error is <= 10 in FP32 units."""
+    a = workloads.normal(n, n, seed).astype(np.float32).astype(np.float64)
+    lu, ipiv, jpiv, info, rep, marg = sfactor(a)
+    assert info == n
+    assert np.array_equal(jpiv, np.arange(n))
+    _, piv, infol = oracle.lapack_sgetrf(a)
+    diff = np.flatnonzero(ipiv != piv)
+    if diff.size:
+        k = int(diff[0])
+        print(f"n={n}: first difference from LAPACK sgetrf at step {k}, margin {marg[k]:.3e}")
+        assert marg[k] < 1e-3, f"pivots differ at step {k} with margin {marg[k]}"
+    b32 = _bwd32(a, lu, ipiv, jpiv, info)
+    print(f"n={n}: FP32 factor backward error {b32:.3f} (FP32 units)", flush=True)
+    assert b32 <= 10.0

@@ -137,11 +137,121 @@ def test_argument_rules():
     assert info == 0 and ipiv.tolist() == [0, 1, 2, 3]
 
 
-def test_planted_mistake_is_caught():
-    """A mutant that breaks ties toward the highest row must fail the long-double check."""
+def test_planted_mistake_is_caught(tmp_path):
+    """A mutant oracle, built from the same source with the column-pivot tie rule flipped to the
+    highest row (DLU_ORACLE_MUTANT_TIES), must disagree with the long-double pin on an input with
+    exact ties, while the real build agrees: the pin can see a wrong tie rule."""
+    path = oracle.build(force=True, out=str(tmp_path / "libdlu_oracle_mutant.so"),
+                        defines=["DLU_ORACLE_MUTANT_TIES"])
+    mutant = oracle.load_build(path)
     a = _tricky(12, 1, ties=True)
-    u, ipu, jpu, infou = oracle.unblocked_ld(a)
-    # mutant: same algorithm, last-index argmax for the first column
-    col = np.abs(a[:, 0])
-    mutant_first = int(np.flatnonzero(col == col.max())[-1])
-    assert mutant_first != ipu[0]
+    _, ipu, jpu, infou = oracle.unblocked_ld(a)
+    _, ipm, jpm, infom, _ = oracle.delayed_getrf(a, library=mutant)
+    _, ipr, jpr, infor, _ = oracle.delayed_getrf(a)
+    assert ipr.tolist() == ipu.tolist() and jpr.tolist() == jpu.tolist()
+    assert ipm.tolist() != ipu.tolist()
+    ties = np.flatnonzero(np.abs(a[:, 0]) == np.abs(a[:, 0]).max())
+    assert len(ties) >= 2 and ipu[0] == ties[0] and ipm[0] == ties[-1]
+
+
+def _frobenius_backward_error(a, lu, ipiv, jpiv, info):
+    """||P A Q - Lbar Ubar||_F / (max(m,n) eps ||A||_F), written independently in numpy."""
+    m, n = a.shape
+    lo = np.eye(m)
+    up = np.zeros((m, n))
+    for k in range(info):
+        lo[k + 1:, k] = lu[k + 1:, k]
+        up[k, k:] = lu[k, k:]
+    up[info:, info:] = lu[info:, info:]
+    rows = oracle.replay(ipiv, info, m)
+    cols = oracle.replay(jpiv, info, n)
+    d = lo @ up - a[np.ix_(rows, cols)]
+    return np.linalg.norm(d) / (max(m, n) * np.finfo(float).eps * np.linalg.norm(a))
+
+
+@pytest.mark.parametrize("case", ["normal", "rect", "stress"])
+def test_backward_error_two_sided(case):
+    """dlu_oracle_backward_error agrees with an independent numpy reconstruction, and reports a
+    large value for a corrupted factor and for a wrong pivot history."""
+    if case == "normal":
+        a = workloads.normal(300, 300, 41)
+    elif case == "rect":
+        a = workloads.normal(260, 190, 42)
+    else:
+        a = workloads.delay_stress(256, 43)[0]
+    lu, ipiv, jpiv, info, _ = oracle.delayed_getrf(a)
+    got = oracle.backward_error(a, lu, ipiv, jpiv, info)
+    ref = _frobenius_backward_error(a, lu, ipiv, jpiv, info)
+    assert 0 <= got < 10
+    # correct factors leave a residual at rounding level, which itself depends on the order in
+    # which L Ubar is formed: the two agree to a fraction of one unit (n eps ||A||_F)
+    assert abs(got - ref) <= 0.1 * max(got, ref) + 0.05
+    bad = lu.copy()
+    bad[info // 2 + 3, info // 3] += 1e-8 * np.abs(lu).max()  # one multiplier off by 1e-8
+    got_bad = oracle.backward_error(a, bad, ipiv, jpiv, info)
+    assert got_bad > 1e3
+    assert abs(got_bad - _frobenius_backward_error(a, bad, ipiv, jpiv, info)) <= 1e-6 * got_bad
+    wrong = ipiv.copy()
+    k = int(np.argmax(wrong != np.arange(len(wrong))))
+    wrong[k] = k  # drop one row interchange
+    assert oracle.backward_error(a, lu, wrong, jpiv, info) > 1e6
+
+
+# ---- the column-delay variant (delayed_lu/delayed_dgetrf_column.cpp) -----------------------
+
+def test_column_variant_known_answer_example_cpp():
+    """example.cpp's 4096 x 4096 rank-800 matrix (diag(2..801) = 800..1): steps 0 and 1 find
+    their columns zero and take the rightmost nonzero columns, 801 then 800; the rest are
+    column pivots; step 800 finds every remaining column zero (info = 800)."""
+    a = workloads.diag_rank(4096, 800)
+    lu, ipiv, jpiv, info, ex = oracle.delayed_getrf(a, column=True)
+    want = [801, 800] + list(range(2, 800)) + list(range(800, 4096))
+    assert info == 800
+    assert ipiv.tolist() == want and jpiv.tolist() == want
+    assert ex["fallbacks"] == 3
+    assert oracle.backward_error(a, lu, ipiv, jpiv, info) < 10
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_column_blocked_matches_unblocked_long_double(seed):
+    n = 40
+    rng = np.random.default_rng(300 + seed)
+    zc = rng.choice(n, size=int(rng.integers(1, 9)), replace=False)
+    a = _tricky(n, seed + 50, zero_cols=zc, ties=seed % 2 == 0, int_entries=seed % 3 != 0)
+    u, ipu, jpu, infou = oracle.unblocked_ld(a, column=True)
+    for nb in (1, 3, 8, 64):
+        lu, ipiv, jpiv, info, _ = oracle.delayed_getrf(a, nb=nb, column=True)
+        assert info == infou
+        assert ipiv.tolist() == ipu.tolist(), f"nb={nb}"
+        assert jpiv.tolist() == jpu.tolist(), f"nb={nb}"
+        assert np.abs(lu - u).max() <= 1e-12 * max(1.0, np.abs(u).max())
+
+
+def test_column_variant_takes_rightmost_column():
+    """Column 0 zero: the full-pivot variant takes the global maximum's column, the column
+    variant the last column (its largest |a|, lowest row on ties)."""
+    n = 30
+    a = _tricky(n, 7, int_entries=True)
+    a[:, 0] = 0.0
+    a[:, n - 1] = 0.0
+    a[4, n - 1] = -2.0
+    a[9, n - 1] = 2.0
+    a[17, 11] = 50.0
+    _, ipf, jpf, _, _ = oracle.delayed_getrf(a)
+    _, ipc, jpc, _, _ = oracle.delayed_getrf(a, column=True)
+    assert (ipf[0], jpf[0]) == (17, 11)
+    assert (ipc[0], jpc[0]) == (4, n - 1)
+
+
+@pytest.mark.parametrize("n,seed", [(512, 3), (1024, 5)])
+def test_column_zero_column_invariant(n, seed):
+    """At tau = 0 the delayed set is the zero-column set for this variant too, and it is reached
+    with one fallback per zero column met (each zero column goes to the right end)."""
+    a, zc, tc, off = workloads.delay_stress(n, seed)
+    lu, ipiv, jpiv, info, ex = oracle.delayed_getrf(a, column=True)
+    assert info == n - len(zc)
+    assert np.array_equal(delayed_set(jpiv, info, n), zc)
+    assert np.all(lu[info:, info:] == 0.0)
+    assert oracle.backward_error(a, lu, ipiv, jpiv, info) < 10
+    _, _, _, _, exf = oracle.delayed_getrf(a)
+    assert ex["fallbacks"] <= len(zc) + 1 < exf["fallbacks"]
