@@ -118,12 +118,19 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libdlu.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64, and a process that loads
+    # libdlu first (which would bring in the system's) cannot then use the GPU through both.  When
+    # PyTorch is installed, it is loaded first and libdlu binds to its runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     sig = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip,
            ctypes.c_void_p]
     for name in ("dlu_getrf_device", "dlu_getrf_host", "dlu_getrf_native_device",
                  "dlu_getrf_native_host", "dlu_getrf_hybrid_device", "dlu_getrf_hybrid_host",
-                 "dlu_getrf_column_device", "dlu_getrf_column_host"):
+                 "dlu_getrf_column_device", "dlu_getrf_column_host", "dlu_getrf_device_noalloc"):
         fn = getattr(L, name)
         fn.argtypes = sig
         fn.restype = ctypes.c_int
