@@ -18,6 +18,8 @@
 
 #include <algorithm>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -267,6 +269,9 @@ struct Run {
   hipStream_t s;  // the caller's stream
   long long spin;
   bool profile = false;
+  // Everything in order on the caller's stream, with no events and no pinned host memory (the
+  // allocation-free entry point); look-ahead, pairs and the panel-stream tail are off then.
+  bool single = false;
   int panels = 0;
   int max_groups = kMaxPanelGroups;  // workgroups the panel stream can hold at once
   int forced_rpl = 0;                // rows per lane requested by the caller (0: automatic)
@@ -345,7 +350,7 @@ struct Run {
     if (e == hipSuccess && profile) e = close(t_panel, st);
     if (e != hipSuccess) return e;
     ++panels;
-    return hipEventRecord(w.ev_ctl, st);
+    return single ? hipSuccess : hipEventRecord(w.ev_ctl, st);
   }
 
   hipError_t panel(int pb, int pe, int positioned, int parity, hipStream_t st) {
@@ -385,7 +390,7 @@ struct Run {
     if (e == hipSuccess && profile) e = close(t_panel, st);
     if (e != hipSuccess) return e;
     ++panels;
-    return hipEventRecord(w.ev_ctl, st);
+    return single ? hipSuccess : hipEventRecord(w.ev_ctl, st);
   }
 
   // The last panel's control block: spin on the sequence number the panel kernel stores into
@@ -393,6 +398,10 @@ struct Run {
   // completes first), the device copy is read instead.
   unsigned long long expect_done = 0;
   hipError_t wait_panel(Control &c) {
+    if (single || w.h_done == nullptr) {  // no pinned copy: read the device's
+      hipError_t e = hipMemcpyAsync(&c, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost, s);
+      return e == hipSuccess ? hipStreamSynchronize(s) : e;
+    }
     for (unsigned it = 1;; ++it) {
       if (__atomic_load_n(&w.h_done->seq, __ATOMIC_ACQUIRE) == expect_done) {
         c = w.h_done->c;
@@ -570,7 +579,8 @@ struct Run {
   // as soon as that block's update is queued: the far work of the step is queued after it, so
   // the host's launch calls for the far work are off the panel chain.
   hipError_t launch_next(int pb_next, int pe_next, int positioned, int parity) {
-    if (tall(pb_next)) return panel(pb_next, pe_next, positioned, (parity + 1) % kLists, us);
+    if (single || tall(pb_next))
+      return panel(pb_next, pe_next, positioned, (parity + 1) % kLists, us);
     hipError_t e = hipEventRecord(w.ev_next, us);
     if (e == hipSuccess) e = hipStreamWaitEvent(ps, w.ev_next, 0);
     if (e == hipSuccess) e = panel(pb_next, pe_next, positioned, (parity + 1) % kLists, ps);
@@ -582,9 +592,11 @@ struct Run {
   // stopped; every pivot decision inside the panel is taken on the device.
   hipError_t factor(int *info, bool lookahead) {
     hipError_t e;
-    if ((e = hipEventRecord(w.ev_fork, s)) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(ps, w.ev_fork, 0)) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(us, w.ev_fork, 0)) != hipSuccess) return e;
+    if (single) lookahead = false;
+    if (!single && ((e = hipEventRecord(w.ev_fork, s)) != hipSuccess ||
+                    (e = hipStreamWaitEvent(ps, w.ev_fork, 0)) != hipSuccess ||
+                    (e = hipStreamWaitEvent(us, w.ev_fork, 0)) != hipSuccess))
+      return e;
     int pb = 0, parity = 0;
     int pe = std::min(pb + nb, steps);
     if ((e = panel(pb, pe, 0, parity, tall(pb) ? us : ps)) != hipSuccess) return e;
@@ -597,7 +609,7 @@ struct Run {
       const int *list = w.d_lists + parity * kListStride;
       const int pe2 = std::min(pe + nb, steps);
       const bool full = !c.deferred && !c.zero_column && wdt == nb && pe - pb == nb;
-      if ((e = hipStreamWaitEvent(us, w.ev_ctl, 0)) != hipSuccess) return e;
+      if (!single && (e = hipStreamWaitEvent(us, w.ev_ctl, 0)) != hipSuccess) return e;
       // A tall next panel interchanges whole rows as it goes: nothing may run beside it.
       const bool ahead = lookahead && !tall(pe);
       const bool second = pair.open && full && pe < steps && ahead;
@@ -705,6 +717,7 @@ struct Run {
 
   // The caller's stream resumes after everything queued on the internal streams.
   hipError_t join() {
+    if (single) return hipSuccess;
     hipError_t e = hipEventRecord(w.ev_join, ps);
     if (e == hipSuccess) e = hipStreamWaitEvent(s, w.ev_join, 0);
     if (e == hipSuccess) e = hipEventRecord(w.ev_join, us);
@@ -754,9 +767,89 @@ void release_pools() {
 
 dlu_status to_status(hipError_t e) {
   if (e == hipSuccess) return DLU_SUCCESS;
+  // DLU_VERBOSE=1 in the environment names the HIP error behind a failed call on stderr.
+  static const bool verbose = [] {
+    const char *v = std::getenv("DLU_VERBOSE");
+    return v != nullptr && v[0] != '\0' && v[0] != '0';
+  }();
+  if (verbose) std::fprintf(stderr, "libdlu: HIP error %d (%s)\n", static_cast<int>(e),
+                            hipGetErrorString(e));
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return DLU_ALLOCATION_FAILED;
   if (e == hipErrorLaunchFailure) return DLU_KERNEL_FAILURE;
   return DLU_RUNTIME_FAILURE;
+}
+
+// The part of a factorization shared by both entry points: control block, swap lists and step
+// arrays initialised on the stream, the factorization, then the pivots, margins and report
+// copied back once the stream has drained.
+template <typename T>
+dlu_status execute(Run<T> &run, bool lookahead, int *info, int *ipiv, int *jpiv,
+                   double *margins, dlu_report *rep) {
+  Workspace &w = run.w;
+  const int steps = run.steps;
+  const hipStream_t stream = run.s;
+  // `hc` stays alive until the stream has drained (its copies are ordered on the stream).
+  Control hc{};
+  hc.min_margin = 2.0;
+  hc.col_found = -1;
+  // Swap lists start empty with their append counters at zero (panels clear them after use;
+  // this also recovers from an aborted call).
+  hipError_t e;
+  if ((e = hipMemcpyAsync(w.d_ctl, &hc, sizeof(Control), hipMemcpyHostToDevice, stream)) ==
+          hipSuccess &&
+      (e = hipMemsetAsync(w.d_lists, 0, kLists * kListStride * sizeof(int), stream)) == hipSuccess &&
+      (e = launch_init_steps(w.d_ipiv, w.d_jpiv, w.d_margins, steps, stream)) == hipSuccess) {
+    int out_info = steps;
+    e = run.factor(&out_info, lookahead);
+    if (e == hipSuccess) *info = out_info;
+  }
+  // Whatever happened, leave no work of this call queued on the internal streams.
+  hipError_t e2 = run.join();
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ipiv, w.d_ipiv, steps * sizeof(int), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(jpiv, w.d_jpiv, steps * sizeof(int), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess && margins)
+    e = hipMemcpyAsync(margins, w.d_margins, steps * sizeof(double), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&hc, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost, stream);
+  e2 = hipStreamSynchronize(stream);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess && hc.error != 0) e = hipErrorLaunchFailure;  // a wait expired late
+  if (e != hipSuccess) {
+    *info = -1;
+    run.release();
+    return to_status(e);
+  }
+  if (rep) {
+    const Control c = hc;
+    rep->panels = run.panels;
+    rep->fallbacks = c.fallbacks;
+    rep->near_ties = c.near_ties;
+    rep->exact_ties = c.exact_ties;
+    rep->min_margin = c.min_margin;
+    if (run.profile) {
+      rep->large_update_launches = static_cast<int>(run.t_update_big.size());
+      rep->large_update_seconds =
+          Run<T>::total(run.t_update_big, &rep->large_update_flops, &rep->large_update_bytes);
+      rep->update_launches = static_cast<int>(run.t_update.size()) + rep->large_update_launches;
+      rep->update_seconds = Run<T>::total(run.t_update, &rep->update_flops) + rep->large_update_seconds;
+      rep->update_flops += rep->large_update_flops;
+      rep->panel_launches = static_cast<int>(run.t_panel.size());
+      rep->panel_seconds = Run<T>::total(run.t_panel, nullptr);
+      rep->solve_launches = static_cast<int>(run.t_solve.size());
+      rep->solve_seconds = Run<T>::total(run.t_solve, nullptr);
+      rep->search_launches = static_cast<int>(run.t_search.size());
+      double search_flops = 0.0;
+      rep->search_seconds = Run<T>::total(run.t_search, &search_flops, &rep->search_bytes);
+      unsigned long long ticks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (hipMemcpy(ticks, w.d_phase, sizeof ticks, hipMemcpyDeviceToHost) == hipSuccess)
+        for (int i = 0; i < 8; ++i) rep->panel_phase_seconds[i] = ticks[i] * 1e-8;  // 100 MHz
+    }
+  }
+  run.release();
+  return DLU_SUCCESS;
 }
 
 template <typename T>
@@ -854,67 +947,98 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
     *info = -1;
     return to_status(e);
   }
-  Control init{};
-  init.min_margin = 2.0;
-  init.col_found = -1;
-  *w.h_ctl = init;
-  // Swap lists start empty with their append counters at zero (panels clear them after use;
-  // this also recovers from an aborted call).
-  if ((e = hipMemcpyAsync(w.d_ctl, w.h_ctl, sizeof(Control), hipMemcpyHostToDevice, stream)) ==
-          hipSuccess &&
-      (e = hipMemsetAsync(w.d_lists, 0, kLists * kListStride * sizeof(int), stream)) == hipSuccess &&
-      (e = launch_init_steps(w.d_ipiv, w.d_jpiv, w.d_margins, steps, stream)) == hipSuccess) {
-    int out_info = steps;
-    e = run.factor(&out_info, lookahead);
-    if (e == hipSuccess) *info = out_info;
-  }
-  // Whatever happened, leave no work of this call queued on the internal streams.
-  hipError_t e2 = run.join();
-  if (e == hipSuccess) e = e2;
+  return execute(run, lookahead, info, ipiv, jpiv, margins, rep);
+}
+
+// ---- The allocation-free entry point --------------------------------------------------------
+// Every device buffer a factorization needs besides the caller's matrix lives in one file-scope
+// device array, laid out below; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (the array's name is not a device address
+// on the host).  The factorization then runs in order on the caller's stream with no events, no
+// pinned host memory and no look-ahead, and reads the control block with a plain copy.
+constexpr int kNoAllocMaxSteps = 32768;  // min(m, n) at most
+constexpr size_t na_align(size_t x) { return (x + 255) / 256 * 256; }
+constexpr size_t kNaCtl = 0;
+constexpr size_t kNaIpiv = kNaCtl + na_align(sizeof(Control));
+constexpr size_t kNaJpiv = kNaIpiv + na_align(sizeof(int) * kNoAllocMaxSteps);
+constexpr size_t kNaMargins = kNaJpiv + na_align(sizeof(int) * kNoAllocMaxSteps);
+constexpr size_t kNaSlots = kNaMargins + na_align(sizeof(double) * kNoAllocMaxSteps);
+constexpr size_t kNaRowk = kNaSlots + na_align(slot_bytes());
+constexpr size_t kNaLists = kNaRowk + na_align(rowk_bytes());
+constexpr size_t kNaPv = kNaLists + na_align(sizeof(int) * kLists * kListStride);
+constexpr size_t kNaPo = kNaPv + na_align(sizeof(double) * kSearchPartials);
+constexpr size_t kNaPhase = kNaPo + na_align(sizeof(unsigned long long) * kSearchPartials);
+constexpr size_t kNoAllocScratchBytes = kNaPhase + na_align(8 * sizeof(unsigned long long));
+
+// About 2.7 MB per device, held until the process ends.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct NoAllocDevice {
+  std::mutex mu;  // calls on one device take turns: they share the array
+  bool ready = false;
+  Workspace w;    // views into this device's copy of the array
+};
+NoAllocDevice g_noalloc[kMaxDevices];
+
+dlu_status getrf_noalloc(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv, int *info,
+                         hipStream_t stream) {
+  if (info == nullptr) return DLU_INVALID_ARGUMENT;
+  *info = -1;
+  if (!valid_arguments(m, n, d_a, lda, ipiv, jpiv)) return DLU_INVALID_ARGUMENT;
+  const int steps = std::min(m, n);
+  if (steps > kNoAllocMaxSteps) return DLU_INVALID_ARGUMENT;  // beyond the array's step arrays
+  *info = steps;
+  for (int k = 0; k < steps; ++k) ipiv[k] = jpiv[k] = k;
+  if (steps == 0) return DLU_SUCCESS;
+  int device = 0, cus = 0;
+  hipError_t e = hipGetDevice(&device);
+  if (e == hipSuccess && (device < 0 || device >= kMaxDevices)) e = hipErrorInvalidDevice;
   if (e == hipSuccess)
-    e = hipMemcpyAsync(ipiv, w.d_ipiv, steps * sizeof(int), hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(jpiv, w.d_jpiv, steps * sizeof(int), hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess && margins)
-    e = hipMemcpyAsync(margins, w.d_margins, steps * sizeof(double), hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost, stream);
-  e2 = hipStreamSynchronize(stream);
-  if (e == hipSuccess) e = e2;
-  if (e == hipSuccess && w.h_ctl->error != 0) e = hipErrorLaunchFailure;  // a wait expired late
-  if (e != hipSuccess) {
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess || cus <= 0) {
     *info = -1;
-    run.release();
-    return to_status(e);
+    return to_status(e != hipSuccess ? e : hipErrorInvalidDevice);
   }
-  if (rep) {
-    const Control c = *w.h_ctl;
-    rep->panels = run.panels;
-    rep->fallbacks = c.fallbacks;
-    rep->near_ties = c.near_ties;
-    rep->exact_ties = c.exact_ties;
-    rep->min_margin = c.min_margin;
-    if (run.profile) {
-      rep->large_update_launches = static_cast<int>(run.t_update_big.size());
-      rep->large_update_seconds =
-          Run<T>::total(run.t_update_big, &rep->large_update_flops, &rep->large_update_bytes);
-      rep->update_launches = static_cast<int>(run.t_update.size()) + rep->large_update_launches;
-      rep->update_seconds = Run<T>::total(run.t_update, &rep->update_flops) + rep->large_update_seconds;
-      rep->update_flops += rep->large_update_flops;
-      rep->panel_launches = static_cast<int>(run.t_panel.size());
-      rep->panel_seconds = Run<T>::total(run.t_panel, nullptr);
-      rep->solve_launches = static_cast<int>(run.t_solve.size());
-      rep->solve_seconds = Run<T>::total(run.t_solve, nullptr);
-      rep->search_launches = static_cast<int>(run.t_search.size());
-      double search_flops = 0.0;
-      rep->search_seconds = Run<T>::total(run.t_search, &search_flops, &rep->search_bytes);
-      unsigned long long ticks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (hipMemcpy(ticks, w.d_phase, sizeof ticks, hipMemcpyDeviceToHost) == hipSuccess)
-        for (int i = 0; i < 8; ++i) rep->panel_phase_seconds[i] = ticks[i] * 1e-8;  // 100 MHz
+  NoAllocDevice &d = g_noalloc[device];
+  std::lock_guard<std::mutex> lk(d.mu);
+  if (!d.ready) {
+    // Once per device, with that device current: this device's address of the array.
+    void *ptr = nullptr;
+    if ((e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch))) != hipSuccess || ptr == nullptr) {
+      *info = -1;
+      if (e == hipSuccess) e = hipErrorInvalidSymbol;  // a null address is no address
+      return to_status(e);
     }
+    char *b = static_cast<char *>(ptr);
+    Workspace &w = d.w;
+    w.device = device;
+    w.cus = cus;
+    w.steps_cap = kNoAllocMaxSteps;
+    w.d_ctl = reinterpret_cast<Control *>(b + kNaCtl);
+    w.d_ipiv = reinterpret_cast<int *>(b + kNaIpiv);
+    w.d_jpiv = reinterpret_cast<int *>(b + kNaJpiv);
+    w.d_margins = reinterpret_cast<double *>(b + kNaMargins);
+    w.d_slots = reinterpret_cast<uint64_t *>(b + kNaSlots);
+    w.d_rowk = reinterpret_cast<uint64_t *>(b + kNaRowk);
+    w.d_lists = reinterpret_cast<int *>(b + kNaLists);
+    w.d_pv = reinterpret_cast<double *>(b + kNaPv);
+    w.d_po = reinterpret_cast<unsigned long long *>(b + kNaPo);
+    w.d_phase = reinterpret_cast<unsigned long long *>(b + kNaPhase);
+    // Exchange words start with step tag 0, which no step uses.
+    if ((e = hipMemsetAsync(w.d_slots, 0, slot_bytes(), stream)) != hipSuccess ||
+        (e = hipMemsetAsync(w.d_rowk, 0, rowk_bytes(), stream)) != hipSuccess) {
+      *info = -1;
+      return to_status(e);
+    }
+    d.ready = true;
   }
-  run.release();
-  return DLU_SUCCESS;
+  Run<double> run{d.w, m, n, lda, steps, 128, d_a, stream, 200000000ll};
+  run.single = true;
+  run.split = false;
+  run.merge_rows = 1 << 30;
+  run.ps = run.us = stream;
+  run.max_groups = std::min(kMaxPanelGroups, cus);
+  return execute(run, false, info, ipiv, jpiv, nullptr, nullptr);
 }
 
 dlu_status getrf_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv, int *info,
@@ -1020,6 +1144,11 @@ dlu_status dlu_getrf_column_host(int m, int n, double *a, int lda, int *ipiv, in
                                  int *info, void *stream) {
   return dlu::getrf_host(m, n, a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream),
                          DLU_FALLBACK_COLUMN);
+}
+
+dlu_status dlu_getrf_device_noalloc(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
+                                    int *info, void *stream) {
+  return dlu::getrf_noalloc(m, n, d_a, lda, ipiv, jpiv, info, static_cast<hipStream_t>(stream));
 }
 
 void dlu_shutdown(void) {

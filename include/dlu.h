@@ -77,6 +77,17 @@ dlu_status dlu_getrf_column_device(int m, int n, double *d_a, int lda, int *ipiv
 dlu_status dlu_getrf_column_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
                                  int *info, void *stream);
 
+/* No counterpart in the reference: dlu_getrf_device for callers that cannot allocate (code
+ * recorded under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and rules as dlu_getrf_device; it calls no allocating routine.  Its scratch
+ * is a fixed device array of about 2.7 MB per GPU, held until the process ends; min(m, n) must be
+ * at most 32768 (else DLU_INVALID_ARGUMENT, info = -1).  The factorization runs in order on
+ * `stream` (no look-ahead, so it is slower than dlu_getrf_device).  Every call on one GPU shares
+ * that GPU's array: the library makes such calls take turns, and each call has finished its
+ * device work when it returns. */
+dlu_status dlu_getrf_device_noalloc(int m, int n, double *d_a, int lda, int *ipiv, int *jpiv,
+                                    int *info, void *stream);
+
 /* No counterpart in the reference (whose calls free everything before returning): frees the
  * device memory, pinned host memory, streams and events libdlu keeps between calls for the
  * calling process's idle factorizations and the calling thread's solves.  Call it before the
