@@ -176,6 +176,13 @@ def lib():
     L.dlu_server_now.restype = ctypes.c_double
     L.dlu_server_destroy.argtypes = [ctypes.c_void_p]
     L.dlu_server_destroy.restype = None
+    L.dlu_front_factor_device.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_int, _ip, _ip, _ip, ctypes.c_void_p]
+    L.dlu_front_factor_device.restype = ctypes.c_int
+    L.dlu_extend_add_device.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    L.dlu_extend_add_device.restype = ctypes.c_int
     L.dlu_shutdown.argtypes = []
     L.dlu_shutdown.restype = None
     _lib = L
@@ -362,6 +369,53 @@ def getrf_host(a: np.ndarray, variant: str = "host", stream=None):
     st = fn(m, n, f.ctypes.data, max(1, m), ipiv.ctypes.data_as(_ip), jpiv.ctypes.data_as(_ip),
             ctypes.byref(info), _stream_ptr(stream))
     return Status(st), f, ipiv[:p], jpiv[:p], info.value
+
+
+def front_factor_device(nf: int, ns: int, d_f: int, ldf: int, stream=None):
+    """Steps 1-2 of a front (dlu_front_factor_device): (status, ipiv, jpiv, info), the pivots
+    being the fully summed block's swap histories (ns entries)."""
+    ipiv = np.zeros(max(ns, 1), dtype=np.int32)
+    jpiv = np.zeros(max(ns, 1), dtype=np.int32)
+    info = ctypes.c_int(-7)
+    st = lib().dlu_front_factor_device(nf, ns, d_f, ldf, ipiv.ctypes.data_as(_ip),
+                                       jpiv.ctypes.data_as(_ip), ctypes.byref(info),
+                                       _stream_ptr(stream))
+    return Status(st), ipiv[:ns], jpiv[:ns], info.value
+
+
+def extend_add_device(rows: int, cols: int, d_child: int, ldc: int, d_map_r: int, d_map_c: int,
+                      d_parent: int, ldp: int, stream=None) -> Status:
+    """parent(map_r[i], map_c[j]) += child(i, j) (dlu_extend_add_device; asynchronous)."""
+    return Status(lib().dlu_extend_add_device(rows, cols, d_child, ldc, d_map_r, d_map_c, d_parent,
+                                              ldp, _stream_ptr(stream)))
+
+
+def contribution_indices(row_ids, col_ids, ns: int, ipiv, jpiv, info: int):
+    """Global row and column indices of a front's contribution block F[info:nf, info:nf] after
+    dlu_front_factor_device (doc Step 3.1: the delayed variables D, then the update variables U,
+    rows and columns kept apart for the unsymmetric case).  row_ids / col_ids are the front's
+    global indices before factoring (fully summed ones first)."""
+    rows = np.array(row_ids, dtype=np.int64)
+    cols = np.array(col_ids, dtype=np.int64)
+    rows[:ns] = rows[:ns][net_order(ipiv, info, ns)]
+    cols[:ns] = cols[:ns][net_order(jpiv, info, ns)]
+    return rows[info:], cols[info:]
+
+
+def parent_positions(parent_ids, child_ids):
+    """Positions of a child's global indices in a parent front's index list (doc Step 3.2-3.3):
+    indices the parent lacks (the child's delayed variables) are appended to it.  Returns
+    (new parent index list, positions)."""
+    ids = list(int(g) for g in parent_ids)
+    where = {g: i for i, g in enumerate(ids)}
+    pos = []
+    for g in child_ids:
+        g = int(g)
+        if g not in where:
+            where[g] = len(ids)
+            ids.append(g)
+        pos.append(where[g])
+    return np.array(ids, dtype=np.int64), np.array(pos, dtype=np.int32)
 
 
 def gemm_sub_device(M, N, K, c_ptr, a_ptr, b_ptr, ld, stream=None) -> Status:

@@ -308,6 +308,35 @@ dlu_status dlu_server_drain(dlu_server *server);
 double dlu_server_now(const dlu_server *server);
 void dlu_server_destroy(dlu_server *server);
 
+/* ---- Frontal-matrix steps (the design note's multifrontal use of the delayed suffix,
+ * doc/（非）对称稠密矩阵的延迟主元LU方法.md:20-440; the reference implements only Step 1, as
+ * delayed_getrf) ------------------------------------------------------------------------------ */
+
+/* Steps 1 and 2 of an nf x nf front (device, column-major, leading dimension ldf) whose leading
+ * ns x ns block is fully summed:
+ *   Step 1: dlu_getrf_device on the fully summed block: P F_SS Q = [L_EE; L_DE] [U_EE U_ED],
+ *           info = the number of eliminated steps e (the fully summed variables [e, ns) are
+ *           delayed; F[e:ns, e:ns] holds their Schur complement S_DD);
+ *   the interchanges extend over the front: the fully summed rows are interchanged across all nf
+ *   columns and the fully summed columns down all nf rows;
+ *   Step 2: U_EU = L_EE^-1 A_EU, L_UE = A_UE U_EE^-1, A_DU -= L_DE U_EU, A_UD -= L_UE U_ED,
+ *           A_UU -= L_UE U_EU (doc :300-317).
+ * On return F holds [[L\U_EE, U_ED, U_EU], [L_DE, S_DD, S_DU], [L_UE, S_UD, S_UU]]: the factors
+ * of the eliminated variables and the contribution block F[e:nf, e:nf] for the parent.  ipiv and
+ * jpiv (HOST, ns entries) are the fully summed block's swap histories, as dlu_getrf_device
+ * returns them.  Synchronous on `stream`. */
+dlu_status dlu_front_factor_device(int nf, int ns, double *d_f, int ldf, int *ipiv, int *jpiv,
+                                   int *info, void *stream);
+
+/* Step 3, the extend-add (doc :360-420): parent(map_r[i], map_c[j]) += child(i, j) for i < rows,
+ * j < cols; child is a rows x cols contribution block (leading dimension ldc), map_r / map_c
+ * (device int arrays) give each of its rows / columns a position in the parent front (leading
+ * dimension ldp); the entries of each map must be distinct.  Several children are added by
+ * successive calls.  Asynchronous on `stream`. */
+dlu_status dlu_extend_add_device(int rows, int cols, const double *d_child, int ldc,
+                                 const int *d_map_r, const int *d_map_c, double *d_parent,
+                                 int ldp, void *stream);
+
 /* Kernel-level entry point of the trailing update: C(MxN) -= A(MxK) * B(KxN), column-major
  * device matrices sharing leading dimension ld.  Asynchronous on `stream`. */
 dlu_status dlu_gemm_sub_device(int M, int N, int K, double *C, const double *A, const double *B,

@@ -228,3 +228,26 @@ def replay(piv, count: int, size: int) -> np.ndarray:
 if __name__ == "__main__":  # manual probe: python oracle/oracle.py
     build(force="--force" in sys.argv)
     print(blas_name())
+
+
+def front_factor(front: np.ndarray, ns: int, nb: int = 64):
+    """Steps 1-2 of a front (doc/（非）对称稠密矩阵的延迟主元LU方法.md:228-317), restated on a copy:
+    delayed_getrf of the leading ns x ns fully summed block (Step 1), its interchanges extended
+    across the front, then U_EU = L_EE^-1 A_EU, L_UE = A_UE U_EE^-1, A_DU -= L_DE U_EU,
+    A_UD -= L_UE U_ED and A_UU -= L_UE U_EU (Step 2).  Returns (F, ipiv, jpiv, info)."""
+    import scipy.linalg
+    f = _fortran(front)
+    nf = f.shape[0]
+    lu, ipiv, jpiv, e, _ = delayed_getrf(f[:ns, :ns], nb=nb)
+    f[:ns, :ns] = lu
+    if nf > ns and e > 0:
+        f[:ns, ns:] = f[:ns, ns:][replay(ipiv, e, ns), :]
+        f[ns:, :ns] = f[ns:, :ns][:, replay(jpiv, e, ns)]
+        l_ee = np.tril(f[:e, :e], -1) + np.eye(e)
+        u_ee = np.triu(f[:e, :e])
+        f[:e, ns:] = scipy.linalg.solve_triangular(l_ee, f[:e, ns:], lower=True,
+                                                   unit_diagonal=True)
+        f[ns:, :e] = scipy.linalg.solve_triangular(u_ee.T, f[ns:, :e].T, lower=True).T
+        f[e:, ns:] -= f[e:, :e] @ f[:e, ns:]
+        f[ns:, e:ns] -= f[ns:, :e] @ f[:e, e:ns]
+    return f, ipiv, jpiv, e

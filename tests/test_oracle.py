@@ -13,7 +13,7 @@ Pins used (each chosen so that a plausible mistake fails one of them):
 import numpy as np
 import pytest
 
-from helpers import golden, oracle, reconstruction_error, sha256, workloads, delayed_set
+from helpers import delayed_set, dlu, golden, oracle, reconstruction_error, sha256, workloads
 
 
 def _case_matrix(c):
@@ -255,3 +255,57 @@ def test_column_zero_column_invariant(n, seed):
     assert oracle.backward_error(a, lu, ipiv, jpiv, info) < 10
     _, _, _, _, exf = oracle.delayed_getrf(a)
     assert ex["fallbacks"] <= len(zc) + 1 < exf["fallbacks"]
+
+
+# ---- frontal-matrix Step 2 and the extend-add index logic -----------------------------------
+
+def _front(nf, ns, seed, zero_fs_cols=()):
+    a = workloads.normal(nf, nf, seed)
+    a[:ns, list(zero_fs_cols)] = 0.0  # zero in the fully summed rows: delayed at tau = 0
+    a[ns:, list(zero_fs_cols)] = 0.0
+    return a
+
+
+@pytest.mark.parametrize("nf,ns,zeros", [(60, 40, ()), (60, 40, (3, 17)), (90, 90, (5,)),
+                                         (50, 20, (0, 19))])
+def test_front_oracle_reconstructs_and_gives_the_schur_complement(nf, ns, zeros):
+    """The front oracle's output is a block LU of the front with the fully summed interchanges
+    (P F Q = Lbar Ubar, backward error <= 10), and its contribution block equals the Schur
+    complement S = F[R, C] - F[R, Ec] F[Er, Ec]^-1 F[Er, C] computed directly (Er / Ec: the
+    eliminated rows / columns, R / C the rest)."""
+    a = _front(nf, ns, nf * 7 + ns, zeros)
+    f, ipiv, jpiv, e, = oracle.front_factor(a, ns)
+    assert e == ns - len(zeros)
+    rows = np.arange(nf)
+    cols = np.arange(nf)
+    rows[:ns] = oracle.replay(ipiv, e, ns)
+    cols[:ns] = oracle.replay(jpiv, e, ns)
+    er, ec = rows[:e], cols[:e]
+    rr, cc = rows[e:], cols[e:]
+    s = a[np.ix_(rr, cc)] - a[np.ix_(rr, ec)] @ np.linalg.solve(a[np.ix_(er, ec)], a[np.ix_(er, cc)])
+    assert np.abs(f[e:, e:] - s).max() <= 1e-10 * max(1.0, np.abs(s).max())
+    # full reconstruction: the front as a compact factorization with e eliminated steps whose
+    # interchanges are those of the fully summed block
+    ip = np.concatenate([ipiv, np.arange(ns, nf)]).astype(np.int32)
+    jp = np.concatenate([jpiv, np.arange(ns, nf)]).astype(np.int32)
+    assert oracle.backward_error(a, f, ip, jp, e) <= 10.0
+
+
+def test_front_oracle_without_update_part_is_delayed_getrf():
+    a = _front(70, 70, 3, (4,))
+    f, ipiv, jpiv, e = oracle.front_factor(a, 70)
+    lu, ip2, jp2, e2, _ = oracle.delayed_getrf(a)
+    assert e == e2 and np.array_equal(ipiv, ip2) and np.array_equal(jpiv, jp2)
+    assert np.array_equal(f, lu)
+
+
+def test_contribution_indices_and_parent_positions():
+    # a front over global variables [10, 11, 12, 13 | 20, 21] (4 fully summed, 2 update)
+    row_ids, col_ids = [10, 11, 12, 13, 20, 21], [10, 11, 12, 13, 20, 21]
+    ipiv = np.array([2, 1, 2, 3], dtype=np.int32)  # step 0 swaps rows 0 and 2
+    jpiv = np.array([3, 1, 2, 3], dtype=np.int32)  # step 0 swaps cols 0 and 3
+    r, c = dlu.contribution_indices(row_ids, col_ids, 4, ipiv, jpiv, 2)
+    assert r.tolist() == [10, 13, 20, 21]  # rows after the swap: 12, 11, 10, 13
+    assert c.tolist() == [12, 10, 20, 21]  # cols after the swap: 13, 11, 12, 10
+    ids, pos = dlu.parent_positions([20, 21, 30], [10, 13, 20, 21])
+    assert ids.tolist() == [20, 21, 30, 10, 13] and pos.tolist() == [3, 4, 0, 1]
