@@ -6,6 +6,7 @@
 //   materialize_partial_panel: the same two steps for an accepted prefix  :101-125
 //   row interchanges span the columns right of the panel                  :202-205
 #include "dlu_internal.hpp"
+#include "dlu_tuning.inc"
 
 namespace dlu {
 namespace {
@@ -395,16 +396,38 @@ hipError_t launch_gemm_tile(T *c, int ldc, const T *a, int lda, const T *b, int 
 
 }  // namespace
 
+namespace {
+
+// The tile configurations of the trailing-update GEMM (workgroup tile, workgroups per CU).
+template <typename T>
+hipError_t launch_gemm_config(int config, T *c, int ldc, const T *a, int lda, const T *b,
+                              int ldb, int M, int N, int K, hipStream_t s) {
+  switch (config) {
+    case 0: return launch_gemm_tile<T, 64, 64, 16, 4>(c, ldc, a, lda, b, ldb, M, N, K, s);
+    case 1: return launch_gemm_tile<T, 128, 128, 16, 2>(c, ldc, a, lda, b, ldb, M, N, K, s);
+    case 2: return launch_gemm_tile<T, 128, 64, 16, 3>(c, ldc, a, lda, b, ldb, M, N, K, s);
+    case 3: return launch_gemm_tile<T, 64, 128, 16, 3>(c, ldc, a, lda, b, ldb, M, N, K, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+int tuned_gemm_config(int M, int N, int K) {
+  const long long tiles = static_cast<long long>((M + 127) / 128) * ((N + 127) / 128);
+  const int size_class = tiles < 64 ? 0 : tiles < 256 ? 1 : tiles < 512 ? 2 : tiles < 2048 ? 3 : 4;
+  return kTunedGemmTiles[K > 160 ? 1 : 0][size_class];
+}
+
+
 template <typename T>
 hipError_t launch_gemm_sub(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
-                           int K, hipStream_t s) {
+                           int K, hipStream_t s, int config) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
-  // Launches of at least 512 tiles of 128 x 128 (the far updates while many rows remain) use
-  // 128 x 128 tiles at two workgroups per CU for their data reuse; smaller launches (the next
-  // panel's block, the last panels) keep 64 x 64 tiles at four per CU, which fill the device.
-  if (gemm_uses_large_tiles(M, N))
-    return launch_gemm_tile<T, 128, 128, 16, 2>(c, ldc, a, lda, b, ldb, M, N, K, s);
-  return launch_gemm_tile<T, 64, 64, 16, 4>(c, ldc, a, lda, b, ldb, M, N, K, s);
+  // The tile configuration comes from the table the autotuner wrote (by depth and launch size)
+  // unless the caller names one.
+  return launch_gemm_config<T>(config >= 0 ? config : tuned_gemm_config(M, N, K), c, ldc, a, lda,
+                               b, ldb, M, N, K, s);
 }
 
 template <typename T>
@@ -439,7 +462,7 @@ hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const 
 
 #define DLU_BLAS3_INSTANTIATE(T)                                                                 \
   template hipError_t launch_gemm_sub<T>(T *, int, const T *, int, const T *, int, int, int, int, \
-                                         hipStream_t);                                           \
+                                         hipStream_t, int);                                      \
   template hipError_t launch_swap_trsm<T>(T *, int, int, int, int, int, const int *, int,        \
                                           hipStream_t);
 DLU_BLAS3_INSTANTIATE(float)

@@ -1169,11 +1169,21 @@ const char *dlu_status_string(dlu_status status) {
 
 dlu_status dlu_gemm_sub_device(int M, int N, int K, double *C, const double *A, const double *B,
                                int ld, void *stream) {
+  return dlu_gemm_sub_device_config(M, N, K, C, A, B, ld, -1, stream);
+}
+
+dlu_status dlu_gemm_sub_device_config(int M, int N, int K, double *C, const double *A,
+                                      const double *B, int ld, int config, void *stream) {
   if (M < 0 || N < 0 || K < 0 || ld < std::max(1, M) || (M > 0 && N > 0 && C == nullptr) ||
-      (M > 0 && K > 0 && A == nullptr) || (K > 0 && N > 0 && B == nullptr) || ld < K)
+      (M > 0 && K > 0 && A == nullptr) || (K > 0 && N > 0 && B == nullptr) || ld < K ||
+      config >= dlu::kGemmConfigs)
     return DLU_INVALID_ARGUMENT;
-  return dlu::to_status(
-      dlu::launch_gemm_sub<double>(C, ld, A, ld, B, ld, M, N, K, static_cast<hipStream_t>(stream)));
+  return dlu::to_status(dlu::launch_gemm_sub<double>(C, ld, A, ld, B, ld, M, N, K,
+                                                     static_cast<hipStream_t>(stream), config));
+}
+
+int dlu_tuned_gemm_config(int M, int N, int K) {
+  return (M < 1 || N < 1 || K < 1) ? -1 : dlu::tuned_gemm_config(M, N, K);
 }
 
 }  // extern "C"

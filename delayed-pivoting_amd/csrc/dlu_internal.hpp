@@ -128,14 +128,19 @@ hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const 
                             int nb, hipStream_t s);
 
 // C(M x N) -= A(M x K) * B(K x N), column-major with leading dimensions ldc, lda, ldb.
-// Large problems use 128 x 128 tiles, everything else 64 x 64 (gemm_uses_large_tiles tells
-// which problems count as large).
+// gemm_uses_large_tiles names the launches the profiled report counts as large (at least 512
+// blocks of 128 x 128 of C: the far updates while many rows remain).
 inline bool gemm_uses_large_tiles(int M, int N) {
   return static_cast<long long>((M + 127) / 128) * ((N + 127) / 128) >= 512;
 }
+// The tile configuration comes from the autotuned table (dlu_tuning.inc) unless `config` names
+// one (0..3: 64x64, 128x128, 128x64, 64x128).
 template <typename T>
 hipError_t launch_gemm_sub(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
-                           int K, hipStream_t s);
+                           int K, hipStream_t s, int config = -1);
+// The configuration the table picks for an M x N x K update.
+int tuned_gemm_config(int M, int N, int K);
+constexpr int kGemmConfigs = 4;
 
 // Row interchanges of the last panel applied to columns [c0, c1) (left of the panel).
 template <typename T>

@@ -12,6 +12,7 @@
 // memory only while it holds none, so the workers cannot wait on each other.
 #include "dlu.h"
 #include "dlu_internal.hpp"
+#include "dlu_tuning.inc"
 
 #include <hip/hip_runtime.h>
 
@@ -509,8 +510,9 @@ dlu_server *dlu_server_create(const dlu_server_config *config) {
     delete s;
     return nullptr;
   }
-  s->cutoff = s->cfg.batch_cutoff > 0 ? std::min(s->cfg.batch_cutoff, DLU_BATCH_MAX_N)
-                                      : s->tune_cutoff();
+  s->cutoff = s->cfg.batch_cutoff > 0    ? std::min(s->cfg.batch_cutoff, DLU_BATCH_MAX_N)
+              : s->cfg.batch_cutoff == 0 ? std::min(dlu::kTunedBatchCutoff, DLU_BATCH_MAX_N)
+                                         : s->tune_cutoff();
   const int workers = s->cfg.large_workers > 0 ? std::min(s->cfg.large_workers, 4) : 3;
   for (int i = 0; i < workers; ++i) {
     hipStream_t st = nullptr;
@@ -594,6 +596,8 @@ void dlu_server_destroy(dlu_server *s) {
   for (hipStream_t st : s->s_large) hipStreamDestroy(st);
   delete s;
 }
+
+int dlu_tuned_batch_cutoff(void) { return std::min(dlu::kTunedBatchCutoff, DLU_BATCH_MAX_N); }
 
 dlu_status dlu_batch_gesv_device(int count, const dlu_batch_item *items, void *stream) {
   if (count < 0 || (count > 0 && !items)) return DLU_INVALID_ARGUMENT;

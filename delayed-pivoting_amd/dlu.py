@@ -141,6 +141,13 @@ def lib():
     L.dlu_gemm_sub_device.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 3 + [
         ctypes.c_int, ctypes.c_void_p]
     L.dlu_gemm_sub_device.restype = ctypes.c_int
+    L.dlu_gemm_sub_device_config.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 3 + [
+        ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    L.dlu_gemm_sub_device_config.restype = ctypes.c_int
+    L.dlu_tuned_gemm_config.argtypes = [ctypes.c_int] * 3
+    L.dlu_tuned_gemm_config.restype = ctypes.c_int
+    L.dlu_tuned_batch_cutoff.argtypes = []
+    L.dlu_tuned_batch_cutoff.restype = ctypes.c_int
     L.dlu_sgetrf_device.argtypes = sig
     L.dlu_sgetrf_device.restype = ctypes.c_int
     L.dlu_sgetrf_device_ex.argtypes = sig + [ctypes.POINTER(Options), ctypes.POINTER(Report), _dp]
@@ -418,9 +425,10 @@ def parent_positions(parent_ids, child_ids):
     return np.array(ids, dtype=np.int64), np.array(pos, dtype=np.int32)
 
 
-def gemm_sub_device(M, N, K, c_ptr, a_ptr, b_ptr, ld, stream=None) -> Status:
-    return Status(lib().dlu_gemm_sub_device(M, N, K, c_ptr, a_ptr, b_ptr, ld,
-                                            _stream_ptr(stream)))
+def gemm_sub_device(M, N, K, c_ptr, a_ptr, b_ptr, ld, stream=None, config=-1) -> Status:
+    """C -= A B on the trailing-update kernel (config: tile configuration, -1 = tuned)."""
+    return Status(lib().dlu_gemm_sub_device_config(M, N, K, c_ptr, a_ptr, b_ptr, ld, config,
+                                                   _stream_ptr(stream)))
 
 
 # ---- torch plumbing: column-major device matrices -------------------------------------------

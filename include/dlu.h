@@ -262,8 +262,9 @@ typedef struct dlu_server dlu_server;
 
 typedef struct dlu_server_config {
   double hbm_budget_gib; /* resident factors + per-request buffers (default 32) */
-  int batch_cutoff;      /* largest n for the batched kernel, <= DLU_BATCH_MAX_N (0: tuned at
-                            creation by timing both paths) */
+  int batch_cutoff;      /* largest n for the batched kernel, <= DLU_BATCH_MAX_N (0: the
+                            autotuned table's value, dlu_tuned_batch_cutoff(); < 0: timed at
+                            creation on this device by running both paths) */
   int mixed;             /* 1: FP32 factor + FP64 refinement above the cutoff; 0: FP64 */
   double tol;            /* refinement target backward error (default 1e-13) */
   int max_batch;         /* most requests per batched launch (default 1024) */
@@ -341,6 +342,17 @@ dlu_status dlu_extend_add_device(int rows, int cols, const double *d_child, int 
  * device matrices sharing leading dimension ld.  Asynchronous on `stream`. */
 dlu_status dlu_gemm_sub_device(int M, int N, int K, double *C, const double *A, const double *B,
                                int ld, void *stream);
+
+/* The same with a named tile configuration (0: 64 x 64, 1: 128 x 128, 2: 128 x 64, 3: 64 x 128;
+ * < 0: the tuned table's choice), for the autotuner and its tests. */
+dlu_status dlu_gemm_sub_device_config(int M, int N, int K, double *C, const double *A,
+                                      const double *B, int ld, int config, void *stream);
+
+/* The tile configuration the autotuned table (delayed-pivoting_amd/csrc/dlu_tuning.inc, written
+ * by delayed-pivoting_amd/autotune.py) picks for an M x N x K trailing update; and the batch
+ * cutoff it holds (the solve service's default). */
+int dlu_tuned_gemm_config(int M, int N, int K);
+int dlu_tuned_batch_cutoff(void);
 
 #ifdef __cplusplus
 }

@@ -100,3 +100,23 @@ def test_empty_problems(m, n):
 def test_gemm_entry_rejects_bad_shapes():
     assert dlu.gemm_sub_device(-1, 2, 2, 1, 1, 1, 4) == dlu.Status.INVALID_ARGUMENT
     assert dlu.gemm_sub_device(8, 2, 2, 1, 1, 1, 4) == dlu.Status.INVALID_ARGUMENT
+
+
+def test_tuned_table_is_what_the_library_uses():
+    """The autotuned table (csrc/dlu_tuning.inc, written by autotune.py) is compiled in: the
+    library's choices equal the table's entries for each depth and size class."""
+    import os
+    import re
+    text = open(os.path.join(os.path.dirname(dlu.LIB_PATH), "..", "csrc", "dlu_tuning.inc")).read()
+    rows = re.search(r"kTunedGemmTiles\[2\]\[5\] = \{(.*)\};", text).group(1)
+    table = [[int(x) for x in re.findall(r"\d+", r)] for r in re.findall(r"\{([^{}]*)\}", rows)]
+    cutoff = int(re.search(r"kTunedBatchCutoff = (\d+);", text).group(1))
+    assert len(table) == 2 and all(len(r) == 5 and all(0 <= c < 4 for c in r) for r in table)
+    L = dlu.lib()
+    sizes = [(896, 896), (1920, 1920), (2816, 2816), (5632, 5632), (16128, 16128)]
+    for d, K in enumerate((128, 256)):
+        for s, (M, N) in enumerate(sizes):
+            assert L.dlu_tuned_gemm_config(M, N, K) == table[d][s], (K, M)
+    assert L.dlu_tuned_batch_cutoff() == min(cutoff, 128) and 32 <= cutoff <= 128
+    assert L.dlu_tuned_gemm_config(0, 5, 5) == -1
+    assert dlu.gemm_sub_device(8, 8, 8, 1, 1, 1, 8, config=4) == dlu.Status.INVALID_ARGUMENT

@@ -67,9 +67,10 @@ def _case_matrix(c):
     return a
 
 
+@pytest.mark.parametrize("config", [-1, 0, 1, 2, 3])
 @pytest.mark.parametrize("M,N,K", [(16, 16, 4), (64, 64, 16), (100, 37, 13), (513, 260, 256),
                                    (1024, 1024, 256), (2000, 3000, 64), (128, 128, 1)])
-def test_gemm_sub_matches_numpy(M, N, K):
+def test_gemm_sub_matches_numpy(M, N, K, config):
     torch = _torch()
     rng = np.random.default_rng(M * 7 + N + K)
     ld = M + 3
@@ -79,7 +80,8 @@ def test_gemm_sub_matches_numpy(M, N, K):
     tc, _ = dlu.to_device(c, ld)
     ta, _ = dlu.to_device(a, ld)
     tb, _ = dlu.to_device(b, ld)
-    st = dlu.gemm_sub_device(M, N, K, tc.data_ptr(), ta.data_ptr(), tb.data_ptr(), ld)
+    st = dlu.gemm_sub_device(M, N, K, tc.data_ptr(), ta.data_ptr(), tb.data_ptr(), ld,
+                             config=config)
     torch.cuda.synchronize()
     assert st == dlu.Status.SUCCESS
     got = dlu.from_device(tc, M)

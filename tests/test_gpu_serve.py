@@ -190,7 +190,7 @@ def _serve(reqs, one_at_a_time, **cfg):
 
 def test_server_result_is_the_same_alone_and_in_the_mix():
     reqs = _requests(3, 40, [40, 128, 300, 96, 700, 64, 1000, 17])
-    c1, x1, cut = _serve(reqs, True, batch_cutoff=0)  # cutoff tuned by the service
+    c1, x1, cut = _serve(reqs, True, batch_cutoff=-1)  # cutoff timed by the service
     c2, x2, _ = _serve(reqs, False, batch_cutoff=cut)
     assert cut >= 32
     assert len(c1) == len(c2) == len(reqs)
@@ -233,10 +233,10 @@ def test_server_mixed_mode_reaches_fp64_backward_error():
 
 
 def test_server_small_max_batch_with_tuned_cutoff():
-    """max_batch below the cutoff tuner's batch (128 matrices) with the cutoff tuned at
-    creation: the tuner's item arrays must still fit, and the service must work."""
+    """max_batch below the start-up cutoff timing's batch (128 matrices) with the cutoff timed at
+    creation: the timing's item arrays must still fit, and the service must work."""
     reqs = _requests(6, 30, [40, 64, 100, 300])
-    comps, xs, cut = _serve(reqs, False, batch_cutoff=0, max_batch=16)
+    comps, xs, cut = _serve(reqs, False, batch_cutoff=-1, max_batch=16)
     assert cut >= 32
     for r in reqs:
         assert comps[r["id"]]["status"] == 0
