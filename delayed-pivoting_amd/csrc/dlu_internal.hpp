@@ -193,12 +193,28 @@ hipError_t launch_gather_rows(const T *src, int lds, T *dst, int ldd, const int 
 // with the epoch of the launch that wrote them; 16 columns x (1 or 2) words per row, rows of the
 // padded right-hand side), a ticket counter that only grows (each launch takes ticket_base ..
 // ticket_base + blocks - 1) and an error word set by an expired wait.
+// A launch's ticket base and epoch in device memory.
+struct ChainClock {
+  unsigned long long base;
+  unsigned epoch;
+  unsigned pad;
+};
+constexpr int kChainSlots = 16;  // chain launches per clock advance
+// Device state of chain launches replayed from a HIP graph: the ticket counter and error word
+// (as in the eager state), the last epoch handed out, and the clocks of the next launches.
+struct ChainClocks {
+  unsigned long long ticket;
+  int err;
+  unsigned epoch;
+  ChainClock slot[kChainSlots];
+};
 struct ChainState {
   unsigned long long *words;
   unsigned long long *ticket;
   unsigned long long ticket_base;
   unsigned epoch;
-  long long spin_limit;  // 100 MHz ticks
+  const ChainClock *clock;  // not null: base and epoch are read from here at run time
+  long long spin_limit;     // 100 MHz ticks
   int *err;
 };
 
@@ -208,6 +224,12 @@ struct ChainState {
 template <typename T>
 hipError_t launch_trsv_chain(const T *lu, int lda, int n, T *x, int ldx, int nrhs, bool upper,
                              const ChainState &cs, hipStream_t s);
+
+// One workgroup: the clocks of the next nslots chain launches of `blocks` tickets each go to
+// st->slot[0, nslots), and zero[0, nzero) is zeroed (a fill kernel, so a HIP graph holds no
+// memset node).  Clears `words` (nwords) when the epochs would wrap.
+hipError_t launch_chain_clock(ChainClocks *st, int nslots, int blocks, unsigned long long *words,
+                              size_t nwords, double *zero, int nzero, hipStream_t s);
 
 // R = B - A X in FP64 (A n x n, X and B n x nrhs).
 hipError_t launch_residual(const double *a, int lda, const double *x, int ldx, const double *b,
@@ -254,6 +276,22 @@ dlu_status solve_refine(const DeviceFactor &f, int nrhs, const double *d_a, int 
                         const double *d_b, int ldb, double *d_x, int ldx, double tol,
                         int max_iter, hipStream_t s, int *iterations, double *bwd,
                         bool *converged);
+// The refinement sweep of solve_refine for one factor, right-hand-side count and matrix,
+// recorded once as a HIP graph with buffers of its own, and replayed for each later solve:
+// b is copied in, the graph runs once per sweep (the host reads the backward errors between
+// sweeps, as solve_refine does), and x is copied out.  Results equal solve_refine's bit for bit
+// (the same kernels on the same values).  The factor and the matrix must stay in place while
+// the plan lives; one solve at a time per plan.
+struct RefinePlan;
+constexpr int kPlanMaxRhs = kChainSlots / 2 * 16;
+dlu_status refine_plan_create(const DeviceFactor &f, int nrhs, const double *d_a, int lda,
+                              hipStream_t s, RefinePlan **out, size_t *bytes);
+bool refine_plan_matches(const RefinePlan *p, int nrhs, const double *d_a, int lda);
+dlu_status refine_plan_solve(RefinePlan *p, const double *d_b, int ldb, double *d_x, int ldx,
+                             double tol, int max_iter, hipStream_t s, int *iterations,
+                             double *bwd, bool *converged);
+// Frees the plan after the work queued on s (the stream of its last use).
+void refine_plan_destroy(RefinePlan *p, hipStream_t s);
 dlu_status to_status(hipError_t e);
 // Process teardown before the HIP runtime goes: the idle factorization workspaces of every
 // device, and the calling thread's solve-side scratch (each serving worker frees its own).

@@ -48,6 +48,9 @@ struct Resident {
   int pins = 0;
   unsigned long long last_use = 0;
   hipStream_t stream = nullptr;  // stream of the last use (frees are ordered after it)
+  RefinePlan *plan = nullptr;    // full path: the recorded refinement sweep of later solves
+  size_t plan_bytes = 0;         // counted in `used` besides `bytes`
+  bool plan_busy = false;        // a worker is solving with the plan (or recording it)
 };
 
 struct Job {
@@ -84,6 +87,8 @@ struct Server {
 
   // ---- memory admission -------------------------------------------------------------------
   void evict_locked(Resident *v) {
+    refine_plan_destroy(v->plan, v->stream);
+    used -= v->plan_bytes;
     hipFreeAsync(v->mem, v->stream);
     used -= v->bytes;
     store.erase(v->id);
@@ -306,6 +311,37 @@ struct Server {
       }
   }
 
+  // The resident factor's recorded sweep for this request (recorded now when the factor has
+  // none for this right-hand-side count and matrix), or null: the plan is in use by another
+  // worker, graphs are off, or the request does not fit one.  *owner: this worker set
+  // plan_busy (it clears it when the request ends).
+  RefinePlan *take_plan(Resident *res, const DeviceFactor &f, const dlu_request &r,
+                        hipStream_t s, bool *owner) {
+    *owner = false;
+    if (cfg.eager_sweeps || r.nrhs < 1 || r.nrhs > kPlanMaxRhs) return nullptr;
+    RefinePlan *old = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (res->plan_busy) return nullptr;
+      res->plan_busy = true;
+      *owner = true;
+      if (refine_plan_matches(res->plan, r.nrhs, r.a, r.lda)) return res->plan;
+      old = res->plan;
+      used -= res->plan_bytes;
+      res->plan = nullptr;
+      res->plan_bytes = 0;
+    }
+    refine_plan_destroy(old, res->stream);  // its last use has finished (requests end synced)
+    RefinePlan *p = nullptr;
+    size_t bytes = 0;
+    if (refine_plan_create(f, r.nrhs, r.a, r.lda, s, &p, &bytes) != DLU_SUCCESS) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    res->plan = p;
+    res->plan_bytes = bytes;
+    used += bytes;
+    return p;
+  }
+
   void large_loop(hipStream_t s_large) {
     hipSetDevice(device);
     size_t charged = 0;
@@ -333,6 +369,7 @@ struct Server {
       c.backward_error = -1.0;
       const int n = r.n;
       Resident *res = r.factor_of >= 0 ? pin(r.factor_of, false) : nullptr;
+      bool plan_owner = false;
       DeviceFactor f;
       void *mem = nullptr;
       size_t bytes = 0;
@@ -381,8 +418,15 @@ struct Server {
         } else {
           std::vector<double> bwd(std::max(r.nrhs, 1), 0.0);
           bool conv = false;
-          st = solve_refine(f, r.nrhs, r.a, r.lda, r.b, r.ldb, r.x, r.ldx, cfg.tol, 30, s_large,
-                            &c.iterations, bwd.data(), &conv);
+          RefinePlan *plan = res ? take_plan(res, f, r, s_large, &plan_owner) : nullptr;
+          if (plan) {
+            st = refine_plan_solve(plan, r.b, r.ldb, r.x, r.ldx, cfg.tol, 30, s_large,
+                                   &c.iterations, bwd.data(), &conv);
+            if (st == DLU_SUCCESS) c.graph_sweeps = c.iterations;
+          } else {
+            st = solve_refine(f, r.nrhs, r.a, r.lda, r.b, r.ldb, r.x, r.ldx, cfg.tol, 30,
+                              s_large, &c.iterations, bwd.data(), &conv);
+          }
           if (st == DLU_SUCCESS) c.backward_error = *std::max_element(bwd.begin(), bwd.end());
         }
       }
@@ -408,6 +452,7 @@ struct Server {
       if (res) {
         res->pins--;
         res->stream = s_large;
+        if (plan_owner) res->plan_busy = false;
       }
       if (kept) keep_locked(kept);
       complete_locked(c);
@@ -582,6 +627,7 @@ void dlu_server_destroy(dlu_server *s) {
   for (std::thread &t : s->t_large)
     if (t.joinable()) t.join();
   for (auto &kv : s->store) {
+    dlu::refine_plan_destroy(kv.second->plan, kv.second->stream);
     hipFreeAsync(kv.second->mem, kv.second->stream);
     delete kv.second;
   }

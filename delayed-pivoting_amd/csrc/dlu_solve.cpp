@@ -103,9 +103,30 @@ bool perms_valid(int n, const int *ipiv, const int *jpiv) {
   return true;
 }
 
-// X = Q U^{-1} L^{-1} P B, in place in b: a gather, one launch per triangle and per 16
-// right-hand sides, a gather.  Scratch (the permuted right-hand sides, rows padded to a
-// multiple of 64, and the chain state) comes from the cache.
+// X = Q U^{-1} L^{-1} P B, in place in b: a gather into work (rows padded to a multiple of
+// 64), one launch per triangle and per 16 right-hand sides, a gather back.  chain(k, cs) fills
+// the state of the k-th triangular-solve launch.
+template <typename T, typename ChainFn>
+hipError_t getrs_chains(int n, int nrhs, const T *lu, int lda, const int *d_prow,
+                        const int *d_pcol, T *b, int ldb, T *work, ChainFn &&chain,
+                        hipStream_t s) {
+  const int npad = (n + kChainRows - 1) / kChainRows * kChainRows;
+  hipError_t e = launch_gather_rows<T>(b, ldb, work, npad, d_prow, n, nrhs, s);
+  int k = 0;
+  for (int c0 = 0; e == hipSuccess && c0 < nrhs; c0 += kChainRhs) {
+    const int nc = std::min(kChainRhs, nrhs - c0);
+    T *xc = work + static_cast<size_t>(c0) * npad;
+    ChainState cs{};
+    if ((e = chain(k++, cs)) == hipSuccess)
+      e = launch_trsv_chain<T>(lu, lda, n, xc, npad, nc, false, cs, s);
+    if (e == hipSuccess && (e = chain(k++, cs)) == hipSuccess)
+      e = launch_trsv_chain<T>(lu, lda, n, xc, npad, nc, true, cs, s);
+  }
+  if (e == hipSuccess) e = launch_gather_rows<T>(work, npad, b, ldb, d_pcol, n, nrhs, s);
+  return e;
+}
+
+// getrs_chains with its scratch (work and the chain state) from the cache.
 template <typename T>
 hipError_t getrs_core(int n, int nrhs, const T *lu, int lda, const int *d_prow, const int *d_pcol,
                       T *b, int ldb, SolveCache &c, hipStream_t s) {
@@ -114,19 +135,8 @@ hipError_t getrs_core(int n, int nrhs, const T *lu, int lda, const int *d_prow, 
   void *pw = nullptr;
   hipError_t e = c.get(SolveCache::WORK, sizeof(T) * static_cast<size_t>(npad) * nrhs, &pw);
   if (e != hipSuccess) return e;
-  T *work = static_cast<T *>(pw);
-  e = launch_gather_rows<T>(b, ldb, work, npad, d_prow, n, nrhs, s);
-  for (int c0 = 0; e == hipSuccess && c0 < nrhs; c0 += kChainRhs) {
-    const int nc = std::min(kChainRhs, nrhs - c0);
-    T *xc = work + static_cast<size_t>(c0) * npad;
-    ChainState cs{};
-    if ((e = c.chain(blocks, cs, s)) == hipSuccess)
-      e = launch_trsv_chain<T>(lu, lda, n, xc, npad, nc, false, cs, s);
-    if (e == hipSuccess && (e = c.chain(blocks, cs, s)) == hipSuccess)
-      e = launch_trsv_chain<T>(lu, lda, n, xc, npad, nc, true, cs, s);
-  }
-  if (e == hipSuccess) e = launch_gather_rows<T>(work, npad, b, ldb, d_pcol, n, nrhs, s);
-  return e;
+  return getrs_chains<T>(n, nrhs, lu, lda, d_prow, d_pcol, b, ldb, static_cast<T *>(pw),
+                         [&](int, ChainState &cs) { return c.chain(blocks, cs, s); }, s);
 }
 
 bool solve_args(int n, int nrhs, const void *lu, int lda, const int *ipiv, const int *jpiv,
@@ -161,31 +171,45 @@ dlu_status getrs_device(int n, int nrhs, const T *d_lu, int lda, const int *ipiv
   return to_status(e);
 }
 
-// Per column j of X: ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf).  norms must
-// hold 3*nrhs+1 doubles and r n*nrhs doubles; normw 32*n doubles.  Results land in host bwd.
+// r = b - A x and the column norms of x, r and b into norms + 1 (zeroed before).
+hipError_t queue_residual_norms(int n, int nrhs, const double *a, int lda, const double *x,
+                                int ldx, const double *b, int ldb, double *r, double *norms,
+                                hipStream_t s) {
+  double *xn = norms + 1, *rn = norms + 1 + nrhs, *bn = norms + 1 + 2 * nrhs;
+  hipError_t e = launch_residual(a, lda, x, ldx, b, ldb, r, n, n, nrhs, s);
+  if (e == hipSuccess) e = launch_norm_inf_columns(x, ldx, n, nrhs, xn, s);
+  if (e == hipSuccess) e = launch_norm_inf_columns(r, n, n, nrhs, rn, s);
+  if (e == hipSuccess) e = launch_norm_inf_columns(b, ldb, n, nrhs, bn, s);
+  return e;
+}
+
+// Per column j of X: ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf), from the
+// norms [||A||, ||x_0..||, ||r_0..||, ||b_0..||] on the host.
+void bwd_from_norms(const double *h, int nrhs, double *bwd) {
+  for (int j = 0; j < nrhs; ++j) {
+    const double den = h[0] * h[1 + j] + h[1 + 2 * nrhs + j];
+    bwd[j] = den > 0.0 ? h[1 + nrhs + j] / den : (h[1 + nrhs + j] > 0.0 ? 1.0 : 0.0);
+  }
+}
+
+// The backward errors of X (see bwd_from_norms).  norms must hold 3*nrhs+1 doubles and r
+// n*nrhs doubles; normw 32*n doubles.  Results land in host bwd.
 hipError_t backward_error(int n, int nrhs, const double *a, int lda, const double *x, int ldx,
                           const double *b, int ldb, double *r, double *normw, double *norms,
                           bool have_anorm, double *bwd, hipStream_t s) {
   hipError_t e = hipSuccess;
-  double *anorm = norms, *xn = norms + 1, *rn = norms + 1 + nrhs, *bn = norms + 1 + 2 * nrhs;
   if (!have_anorm) {
-    e = hipMemsetAsync(anorm, 0, sizeof(double), s);
-    if (e == hipSuccess) e = launch_norm_inf_matrix(a, lda, n, normw, anorm, s);
+    e = hipMemsetAsync(norms, 0, sizeof(double), s);
+    if (e == hipSuccess) e = launch_norm_inf_matrix(a, lda, n, normw, norms, s);
   }
-  if (e == hipSuccess) e = hipMemsetAsync(xn, 0, 3 * sizeof(double) * nrhs, s);
-  if (e == hipSuccess) e = launch_residual(a, lda, x, ldx, b, ldb, r, n, n, nrhs, s);
-  if (e == hipSuccess) e = launch_norm_inf_columns(x, ldx, n, nrhs, xn, s);
-  if (e == hipSuccess) e = launch_norm_inf_columns(r, n, n, nrhs, rn, s);
-  if (e == hipSuccess) e = launch_norm_inf_columns(b, ldb, n, nrhs, bn, s);
+  if (e == hipSuccess) e = hipMemsetAsync(norms + 1, 0, 3 * sizeof(double) * nrhs, s);
+  if (e == hipSuccess) e = queue_residual_norms(n, nrhs, a, lda, x, ldx, b, ldb, r, norms, s);
   std::vector<double> h(3 * static_cast<size_t>(nrhs) + 1);
   if (e == hipSuccess)
     e = hipMemcpyAsync(h.data(), norms, h.size() * sizeof(double), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return e;
-  for (int j = 0; j < nrhs; ++j) {
-    const double den = h[0] * h[1 + j] + h[1 + 2 * nrhs + j];
-    bwd[j] = den > 0.0 ? h[1 + nrhs + j] / den : (h[1 + nrhs + j] > 0.0 ? 1.0 : 0.0);
-  }
+  bwd_from_norms(h.data(), nrhs, bwd);
   return hipSuccess;
 }
 
@@ -324,6 +348,200 @@ dlu_status solve_refine(const DeviceFactor &f, int nrhs, const double *d_a, int 
   bool failed = false;
   if ((e = c.chain_error(s, &failed)) != hipSuccess) return to_status(e);
   if (failed) return DLU_KERNEL_FAILURE;
+  if (iterations) *iterations = it;
+  if (converged) *converged = conv;
+  if (bwd)
+    for (int j = 0; j < nrhs; ++j) bwd[j] = be[j];
+  return DLU_SUCCESS;
+}
+
+// ---- refinement sweeps replayed from a HIP graph ------------------------------------------
+
+struct RefinePlan {
+  int n = 0, nrhs = 0;
+  bool fp32 = false;
+  const void *lu = nullptr;
+  const int *d_perm = nullptr;
+  const double *a = nullptr;
+  int lda = 0;
+  void *mem = nullptr;  // one stream-ordered allocation holds every buffer below
+  size_t bytes = 0;
+  ChainClocks *st = nullptr;  // followed by the norms: one device-to-host copy reads both
+  double *norms = nullptr;
+  size_t report_bytes = 0;    // st .. the end of the norms
+  double *B = nullptr, *X = nullptr, *R = nullptr;  // n x nrhs each, leading dimension n
+  float *R32 = nullptr;
+  void *work = nullptr;       // npad x nrhs of the factor's type
+  unsigned long long *words = nullptr;
+  size_t nwords = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<char> host;     // the report on the host
+  bool broken = false;        // a wait expired: the counter may have drifted, never replay
+};
+
+namespace {
+
+// One sweep, as solve_refine issues it, onto a capturing stream.
+hipError_t record_sweep(RefinePlan &p, hipStream_t s) {
+  const int n = p.n, nrhs = p.nrhs;
+  const int blocks = (n + kChainRows - 1) / kChainRows;
+  const int slots = 2 * ((nrhs + kChainRhs - 1) / kChainRhs);
+  auto chain = [&](int k, ChainState &cs) {
+    cs.words = p.words;
+    cs.ticket = &p.st->ticket;
+    cs.err = &p.st->err;
+    cs.clock = &p.st->slot[k];
+    cs.spin_limit = kChainSpin;
+    return k < slots ? hipSuccess : hipErrorInvalidValue;
+  };
+  hipError_t e = launch_chain_clock(p.st, slots, blocks, p.words, p.nwords, p.norms + 1,
+                                    3 * nrhs, s);
+  if (e != hipSuccess) return e;
+  if (p.fp32) {
+    e = launch_to_float(p.R, n, p.R32, n, n, nrhs, s);
+    if (e == hipSuccess)
+      e = getrs_chains<float>(n, nrhs, static_cast<const float *>(p.lu), n, p.d_perm,
+                              p.d_perm + n, p.R32, n, static_cast<float *>(p.work), chain, s);
+    if (e == hipSuccess) e = launch_add_correction<float>(p.X, n, p.R32, n, n, nrhs, s);
+  } else {
+    e = getrs_chains<double>(n, nrhs, static_cast<const double *>(p.lu), n, p.d_perm,
+                             p.d_perm + n, p.R, n, static_cast<double *>(p.work), chain, s);
+    if (e == hipSuccess) e = launch_add_correction<double>(p.X, n, p.R, n, n, nrhs, s);
+  }
+  if (e == hipSuccess)
+    e = queue_residual_norms(n, nrhs, p.a, p.lda, p.X, n, p.B, n, p.R, p.norms, s);
+  return e;
+}
+
+}  // namespace
+
+bool refine_plan_matches(const RefinePlan *p, int nrhs, const double *d_a, int lda) {
+  return p && !p->broken && p->nrhs == nrhs && p->a == d_a && p->lda == lda;
+}
+
+void refine_plan_destroy(RefinePlan *p, hipStream_t s) {
+  if (!p) return;
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  if (p->mem) hipFreeAsync(p->mem, s);
+  delete p;
+}
+
+dlu_status refine_plan_create(const DeviceFactor &f, int nrhs, const double *d_a, int lda,
+                              hipStream_t s, RefinePlan **out, size_t *bytes) {
+  *out = nullptr;
+  const int n = f.n;
+  if (f.info != n || n < 1 || nrhs < 1 || nrhs > kPlanMaxRhs || !d_a || lda < n)
+    return DLU_INVALID_ARGUMENT;
+  RefinePlan *p = new RefinePlan;
+  p->n = n;
+  p->nrhs = nrhs;
+  p->fp32 = f.fp32;
+  p->lu = f.lu;
+  p->d_perm = f.d_perm;
+  p->a = d_a;
+  p->lda = lda;
+  const size_t nr = static_cast<size_t>(n) * nrhs;
+  const int npad = (n + kChainRows - 1) / kChainRows * kChainRows;
+  const int blocks = npad / kChainRows;
+  auto al = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+  const size_t o_norms = al(sizeof(ChainClocks));
+  p->report_bytes = o_norms + sizeof(double) * (3 * static_cast<size_t>(nrhs) + 1);
+  const size_t o_b = al(p->report_bytes);
+  const size_t o_x = o_b + al(nr * sizeof(double));
+  const size_t o_r = o_x + al(nr * sizeof(double));
+  const size_t o_r32 = o_r + al(nr * sizeof(double));
+  const size_t o_work = o_r32 + (f.fp32 ? al(nr * sizeof(float)) : 0);
+  p->nwords = static_cast<size_t>(32) * 64 * blocks;
+  const size_t o_words =
+      o_work + al(static_cast<size_t>(npad) * nrhs * (f.fp32 ? sizeof(float) : sizeof(double)));
+  p->bytes = o_words + al(p->nwords * sizeof(unsigned long long));
+  double *normw = nullptr;  // the matrix norm's scratch, for the creation only
+  hipError_t e = hipMallocAsync(&p->mem, p->bytes, s);
+  if (e == hipSuccess) {
+    e = hipMallocAsync(reinterpret_cast<void **>(&normw), 32 * sizeof(double) * n, s);
+    if (e != hipSuccess) hipFreeAsync(p->mem, s);
+  }
+  if (e != hipSuccess) {
+    delete p;
+    return DLU_ALLOCATION_FAILED;
+  }
+  char *m = static_cast<char *>(p->mem);
+  p->st = reinterpret_cast<ChainClocks *>(m);
+  p->norms = reinterpret_cast<double *>(m + o_norms);
+  p->B = reinterpret_cast<double *>(m + o_b);
+  p->X = reinterpret_cast<double *>(m + o_x);
+  p->R = reinterpret_cast<double *>(m + o_r);
+  p->R32 = f.fp32 ? reinterpret_cast<float *>(m + o_r32) : nullptr;
+  p->work = m + o_work;
+  p->words = reinterpret_cast<unsigned long long *>(m + o_words);
+  p->host.resize(p->report_bytes);
+  // Eager set-up on s: the counter, epochs and words start at zero, ||A|| goes to norms[0].
+  e = hipMemsetAsync(m, 0, p->report_bytes, s);
+  if (e == hipSuccess) e = hipMemsetAsync(p->words, 0, p->nwords * sizeof(unsigned long long), s);
+  if (e == hipSuccess) e = launch_norm_inf_matrix(d_a, lda, n, normw, p->norms, s);
+  const hipError_t ef = hipFreeAsync(normw, s);
+  if (e == hipSuccess) e = ef;
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  // The sweep, recorded on a stream of its own (the plan's graph can then run on any stream).
+  hipStream_t cap = nullptr;
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      const hipError_t er = record_sweep(*p, cap);
+      e = hipStreamEndCapture(cap, &p->graph);
+      if (er != hipSuccess) e = er;
+    }
+    hipStreamDestroy(cap);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    refine_plan_destroy(p, s);
+    return to_status(e);
+  }
+  *out = p;
+  if (bytes) *bytes = p->bytes;
+  return DLU_SUCCESS;
+}
+
+dlu_status refine_plan_solve(RefinePlan *p, const double *d_b, int ldb, double *d_x, int ldx,
+                             double tol, int max_iter, hipStream_t s, int *iterations,
+                             double *bwd, bool *converged) {
+  if (!p || p->broken || !d_b || !d_x || ldb < p->n || ldx < p->n) return DLU_INVALID_ARGUMENT;
+  const int n = p->n, nrhs = p->nrhs;
+  const size_t col = static_cast<size_t>(n) * sizeof(double);
+  hipError_t e = hipMemcpy2DAsync(p->B, col, d_b, ldb * sizeof(double), col, nrhs,
+                                  hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(p->X, 0, col * nrhs, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(p->R, p->B, col * nrhs, hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return to_status(e);
+  std::vector<double> be(nrhs, 1.0), h(3 * static_cast<size_t>(nrhs) + 1);
+  const size_t o_norms = reinterpret_cast<char *>(p->norms) - reinterpret_cast<char *>(p->st);
+  int it = 0;
+  bool conv = false;
+  while (it < max_iter) {
+    ++it;
+    if ((e = hipGraphLaunch(p->exec, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(p->host.data(), p->st, p->report_bytes, hipMemcpyDeviceToHost, s)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+      return to_status(e);
+    if (reinterpret_cast<const ChainClocks *>(p->host.data())->err != 0) {
+      p->broken = true;
+      return DLU_KERNEL_FAILURE;
+    }
+    std::memcpy(h.data(), p->host.data() + o_norms, h.size() * sizeof(double));
+    bwd_from_norms(h.data(), nrhs, be.data());
+    if (*std::max_element(be.begin(), be.end()) <= tol) {
+      conv = true;
+      break;
+    }
+  }
+  e = hipMemcpy2DAsync(d_x, ldx * sizeof(double), p->X, col, col, nrhs, hipMemcpyDeviceToDevice,
+                       s);
+  if (e != hipSuccess) return to_status(e);
   if (iterations) *iterations = it;
   if (converged) *converged = conv;
   if (bwd)

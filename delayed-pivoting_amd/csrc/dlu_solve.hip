@@ -46,6 +46,10 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const T *__restrict__ 
 // alone, in registers, and publishes the rows.  Rows at and beyond n are zero before the
 // diagonal solve and stay zero.  Waits are bounded: after spin_limit ticks of the 100 MHz
 // clock a wave sets *err and goes on.
+//
+// The launch's ticket base and epoch come from the arguments, or, when clock is not null, from
+// device memory (written by chain_clock_kernel earlier on the stream): a launch recorded in a
+// HIP graph then takes fresh values on every replay.
 // ---------------------------------------------------------------------------------------------
 template <typename T>
 struct ChainWords;
@@ -72,7 +76,8 @@ template <typename T, bool UPPER>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(
     const T *__restrict__ lu, int lda, int n, T *__restrict__ x, int ldx, int nrhs,
     unsigned long long *__restrict__ xw, unsigned long long *__restrict__ ticket,
-    unsigned long long ticket_base, unsigned epoch, long long spin_limit, int *__restrict__ err) {
+    unsigned long long ticket_base_arg, unsigned epoch_arg, const ChainClock *__restrict__ clock,
+    long long spin_limit, int *__restrict__ err) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
   typedef ChainWords<T> CW;
@@ -83,7 +88,9 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nblk = (n + 63) / 64;
+  const unsigned epoch = clock ? clock->epoch : epoch_arg;
   if (t == 0) {
+    const unsigned long long ticket_base = clock ? clock->base : ticket_base_arg;
     const unsigned long long tk = atomicAdd(ticket, 1ull) - ticket_base;
     s_blk = tk < static_cast<unsigned long long>(nblk) ? static_cast<int>(tk) : -1;
     if (s_blk < 0) atomicExch(err, 2);  // the counter and the host's base disagree
@@ -374,6 +381,31 @@ __global__ __launch_bounds__(256) void add_correction_kernel(double *__restrict_
   if (i < n) x[i + j * ldx] += static_cast<double>(d[i + j * ldd]);
 }
 
+// Hands out the ticket bases and epochs of the next nslots chain launches (each of `blocks`
+// tickets) and zeroes zero[0, nzero).  Every earlier chain launch on the stream has finished,
+// so the counter holds its final value and no reader is left on the words.  When the epochs
+// would wrap, the words are cleared and the epochs start again from 1.
+__global__ __launch_bounds__(256) void chain_clock_kernel(ChainClocks *st, int nslots, int blocks,
+                                                          unsigned long long *words,
+                                                          size_t nwords, double *zero,
+                                                          int nzero) {
+  const unsigned e0 = st->epoch;
+  const unsigned long long t0 = st->ticket;
+  const bool wrap = e0 > 0xFFFFFFFFu - static_cast<unsigned>(nslots);
+  if (wrap)
+    for (size_t i = threadIdx.x; i < nwords; i += 256) words[i] = 0ull;
+  for (int i = threadIdx.x; i < nzero; i += 256) zero[i] = 0.0;
+  __syncthreads();  // every thread has read st->epoch before thread 0 moves it on
+  if (threadIdx.x == 0) {
+    const unsigned e = wrap ? 0u : e0;
+    for (int j = 0; j < nslots; ++j) {
+      st->slot[j].base = t0 + static_cast<unsigned long long>(j) * blocks;
+      st->slot[j].epoch = e + 1u + static_cast<unsigned>(j);
+    }
+    st->epoch = e + static_cast<unsigned>(nslots);
+  }
+}
+
 }  // namespace
 
 template <typename T>
@@ -393,12 +425,20 @@ hipError_t launch_trsv_chain(const T *lu, int lda, int n, T *x, int ldx, int nrh
   const int nblk = (n + 63) / 64;
   if (upper)
     hipLaunchKernelGGL((trsv_chain_kernel<T, true>), dim3(nblk), dim3(256), 0, s, lu, lda, n, x,
-                       ldx, nrhs, cs.words, cs.ticket, cs.ticket_base, cs.epoch, cs.spin_limit,
-                       cs.err);
+                       ldx, nrhs, cs.words, cs.ticket, cs.ticket_base, cs.epoch, cs.clock,
+                       cs.spin_limit, cs.err);
   else
     hipLaunchKernelGGL((trsv_chain_kernel<T, false>), dim3(nblk), dim3(256), 0, s, lu, lda, n, x,
-                       ldx, nrhs, cs.words, cs.ticket, cs.ticket_base, cs.epoch, cs.spin_limit,
-                       cs.err);
+                       ldx, nrhs, cs.words, cs.ticket, cs.ticket_base, cs.epoch, cs.clock,
+                       cs.spin_limit, cs.err);
+  return hipGetLastError();
+}
+
+hipError_t launch_chain_clock(ChainClocks *st, int nslots, int blocks, unsigned long long *words,
+                              size_t nwords, double *zero, int nzero, hipStream_t s) {
+  if (nslots < 0 || nslots > kChainSlots || blocks < 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(chain_clock_kernel, dim3(1), dim3(256), 0, s, st, nslots, blocks, words,
+                     nwords, zero, nzero);
   return hipGetLastError();
 }
 

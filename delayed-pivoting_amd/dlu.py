@@ -86,7 +86,7 @@ class BatchItem(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_gib", ctypes.c_double), ("batch_cutoff", ctypes.c_int),
                 ("mixed", ctypes.c_int), ("tol", ctypes.c_double), ("max_batch", ctypes.c_int),
-                ("large_workers", ctypes.c_int)]
+                ("large_workers", ctypes.c_int), ("eager_sweeps", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -100,7 +100,8 @@ class Completion(ctypes.Structure):
     _fields_ = [("id", ctypes.c_longlong), ("status", ctypes.c_int), ("info", ctypes.c_int),
                 ("path", ctypes.c_int), ("iterations", ctypes.c_int), ("reused", ctypes.c_int),
                 ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
-                ("t_done", ctypes.c_double), ("backward_error", ctypes.c_double)]
+                ("t_done", ctypes.c_double), ("backward_error", ctypes.c_double),
+                ("graph_sweeps", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -224,9 +225,9 @@ class Server:
     """Thin wrapper of dlu_server_*: submit requests, poll completions, drain, close."""
 
     def __init__(self, budget_gib=32.0, batch_cutoff=0, mixed=False, tol=1e-13, max_batch=1024,
-                 large_workers=0):
+                 large_workers=0, graphs=True):
         cfg = ServerConfig(budget_gib, batch_cutoff, 1 if mixed else 0, tol, max_batch,
-                           large_workers)
+                           large_workers, 0 if graphs else 1)
         self._h = lib().dlu_server_create(ctypes.byref(cfg))
         if not self._h:
             raise RuntimeError("dlu_server_create failed")

@@ -270,6 +270,10 @@ typedef struct dlu_server_config {
   int max_batch;         /* most requests per batched launch (default 1024) */
   int large_workers;     /* requests above the cutoff factored at once, each on its own stream
                             (default 3, at most 4) */
+  int eager_sweeps;      /* 0 (default): a later right-hand side of a resident factor (above
+                            the cutoff, nrhs <= 128) replays the factor's refinement sweep from a
+                            HIP graph recorded at its first reuse; 1: every sweep is launched
+                            kernel by kernel.  The results are the same bit for bit. */
 } dlu_server_config;
 
 /* The service's streams are not ordered after the caller's: a, b and x must hold their final
@@ -297,6 +301,7 @@ typedef struct dlu_completion {
   int reused;            /* 1 when a resident factor served the request */
   double t_submit, t_start, t_done;  /* seconds on the server clock (dlu_server_now) */
   double backward_error; /* largest over the columns (full paths; -1 when not computed) */
+  int graph_sweeps;      /* refinement sweeps replayed from a HIP graph (0: launched eagerly) */
 } dlu_completion;
 
 dlu_server *dlu_server_create(const dlu_server_config *config);

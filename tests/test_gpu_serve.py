@@ -7,7 +7,8 @@
     factors exactly;
   * the service: each request's solution is bitwise the same submitted alone and in a mix of
     small and large, fresh and reused requests; evicted factors are rebuilt; the mixed mode
-    reaches the FP64 backward error.
+    reaches the FP64 backward error; refinement sweeps replayed from HIP graphs give the same
+    bits as sweeps launched kernel by kernel.
 """
 import numpy as np
 import pytest
@@ -262,3 +263,58 @@ def test_server_create_destroy_returns_device_memory():
     free = [cycle() for _ in range(4)]
     print("free bytes after each cycle:", base, free, flush=True)
     assert min(free) >= base - (16 << 20)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_server_graph_replayed_sweeps_equal_eager_sweeps(mixed):
+    """Later right-hand sides of a resident factor replay the factor's refinement sweep from a
+    HIP graph recorded at its first reuse.  Against the same requests with every sweep launched
+    kernel by kernel: every solution bitwise equal and the same sweep counts.  Each factor's
+    graph is replayed by several requests with other work between the replays (fresh
+    factorizations, batched rounds, the other factors' graphs); a request with another nrhs
+    records the graph again; run once request by request (every reuse replays the graph) and
+    once all at once (concurrent reuses of one factor take turns or sweep eagerly)."""
+    rng = np.random.default_rng(91)
+    sizes = [300, 1000, 2100]
+    reqs, fid = [], {}
+
+    def add(n, nrhs, reuse, keep, a=None):
+        i = len(reqs)
+        if a is None:
+            a = reqs[reuse]["a"] if reuse >= 0 else workloads.normal(n, n, 4000 + i)
+        reqs.append(dict(id=i, n=n, nrhs=nrhs, a=a, reuse=reuse, keep=keep,
+                         b=np.asfortranarray(rng.standard_normal((n, nrhs)))))
+        return i
+
+    for n in sizes:
+        fid[n] = add(n, 2, -1, True)
+    for rnd in range(4):
+        for n in sizes:
+            add(n, 5 if rnd == 2 else 2, fid[n], False)
+        add(500 + 10 * rnd, 3, -1, False)  # a fresh factorization between the replays
+        add(64, 2, -1, False)              # and a batched round
+    ce, xe, _ = _serve(reqs, True, batch_cutoff=128, mixed=mixed, graphs=False)
+    cg, xg, _ = _serve(reqs, True, batch_cutoff=128, mixed=mixed, graphs=True)
+    cc, xc, _ = _serve(reqs, False, batch_cutoff=128, mixed=mixed, graphs=True)
+    replays = 0
+    for r in reqs:
+        i = r["id"]
+        for c in (ce[i], cg[i], cc[i]):
+            assert c["status"] == 0, (i, c)
+        assert np.array_equal(xe[i], xg[i]), f"request {i} (n={r['n']}): graph != eager"
+        assert np.array_equal(xe[i], xc[i]), f"request {i} (n={r['n']}): concurrent != eager"
+        assert ce[i]["iterations"] == cg[i]["iterations"] == cc[i]["iterations"], i
+        assert ce[i]["graph_sweeps"] == 0
+        if r["reuse"] >= 0:
+            assert cg[i]["reused"] == 1
+            assert cg[i]["graph_sweeps"] == cg[i]["iterations"] >= 1, (i, cg[i])
+            replays += 1
+        else:
+            assert cg[i]["graph_sweeps"] == 0
+        x = xe[i]
+        res = np.abs(r["b"] - r["a"] @ x).max(axis=0)
+        den = np.abs(r["a"]).sum(axis=1).max() * np.abs(x).max(axis=0) + np.abs(r["b"]).max(axis=0)
+        assert (res / den).max() <= 1e-13, i
+    assert replays == 4 * len(sizes)
+    print(f"mixed={mixed}: {replays} graph-replayed requests, sweeps per request "
+          f"{sorted({cg[r['id']]['iterations'] for r in reqs if r['reuse'] >= 0})}", flush=True)
