@@ -120,3 +120,33 @@ def test_tuned_table_is_what_the_library_uses():
     assert L.dlu_tuned_batch_cutoff() == min(cutoff, 128) and 32 <= cutoff <= 128
     assert L.dlu_tuned_gemm_config(0, 5, 5) == -1
     assert dlu.gemm_sub_device(8, 8, 8, 1, 1, 1, 8, config=4) == dlu.Status.INVALID_ARGUMENT
+
+
+def test_python_struct_layouts_match_the_header(tmp_path):
+    """Every ctypes mirror of a dlu.h struct has the C struct's size and field offsets (gcc on
+    the header itself)."""
+    import os
+    mirrors = {"dlu_options": dlu.Options, "dlu_report": dlu.Report,
+               "dlu_mixed_options": dlu.MixedOptions, "dlu_mixed_report": dlu.MixedReport,
+               "dlu_batch_item": dlu.BatchItem, "dlu_server_config": dlu.ServerConfig,
+               "dlu_request": dlu.Request, "dlu_completion": dlu.Completion}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dlu.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    include = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", include, str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                               check=True).stdout.splitlines():
+        c, f, v = line.split()
+        got[(c, f)] = int(v)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
