@@ -202,10 +202,14 @@ def bench_sweep(dlu, torch, dev, sizes):
 
 def bench_serving(dlu, torch, dev, count, seed=20000):
     """Poisson arrivals of factor + solve requests, n log-uniform in [64, 4096], 1-8 RHS,
-    30% reusing an earlier request's resident factor; FP64 and mixed-precision services."""
+    30% reusing an earlier request's resident factor; FP64 and mixed-precision services.
+    Each new request's matrix is its own window (leading dimension 4096) of one N(0,1) pool of
+    4096 x 32768 doubles at a random row and column offset: the service only reads A, and
+    20000 separate matrices would need about 226 GB."""
     import math
     import numpy as np
     rng = np.random.default_rng(seed)
+    ld, pool_cols = 4096, 32768
     reqs = []
     for i in range(count):
         if i >= 8 and rng.random() < 0.3:
@@ -215,14 +219,14 @@ def bench_serving(dlu, torch, dev, count, seed=20000):
             reqs.append({"n": reqs[j]["n"], "reuse": j, "nrhs": int(rng.integers(1, 9))})
         else:
             n = int(round(math.exp(rng.uniform(math.log(64), math.log(4096)))))
-            reqs.append({"n": n, "reuse": -1, "nrhs": int(rng.integers(1, 9))})
+            reqs.append({"n": n, "reuse": -1, "nrhs": int(rng.integers(1, 9)),
+                         "off": int(rng.integers(0, ld - n + 1))
+                         + ld * int(rng.integers(0, pool_cols - n + 1))})
     log(f"serving: {count} requests, generating inputs on the device")
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
-    mats = {}
-    for i, r in enumerate(reqs):
-        if r["reuse"] < 0:
-            mats[i] = torch.randn((r["n"], r["n"]), dtype=torch.float64, device=dev, generator=g)
+    pool = torch.randn((pool_cols, ld), dtype=torch.float64, device=dev, generator=g)
+    for r in reqs:
         r["b"] = torch.randn((r["nrhs"], r["n"]), dtype=torch.float64, device=dev, generator=g)
         r["x"] = torch.zeros_like(r["b"])
     torch.cuda.synchronize()
@@ -230,32 +234,38 @@ def bench_serving(dlu, torch, dev, count, seed=20000):
                 for r in reqs)
 
     def a_of(i):
-        return mats[reqs[i]["reuse"] if reqs[i]["reuse"] >= 0 else i]
+        """Device address of request i's matrix (leading dimension ld)."""
+        owner = reqs[i]["reuse"] if reqs[i]["reuse"] >= 0 else i
+        return pool.data_ptr() + 8 * reqs[owner]["off"]
 
-    def run(srv, arrivals):
+    def run(srv, arrivals, upto=count):
         t0 = srv.now()
-        for i, r in enumerate(reqs):
+        for i, r in enumerate(reqs[:upto]):
             if arrivals is not None:
                 while srv.now() - t0 < arrivals[i]:
                     time.sleep(min(0.0005, max(0.0, arrivals[i] - (srv.now() - t0))))
             n = r["n"]
-            st = srv.submit(i, n, r["nrhs"], a_of(i).data_ptr(), n, r["b"].data_ptr(), n,
+            st = srv.submit(i, n, r["nrhs"], a_of(i), ld, r["b"].data_ptr(), n,
                             r["x"].data_ptr(), n, factor_of=r["reuse"], keep=r["reuse"] < 0)
             if st != dlu.Status.SUCCESS:
                 raise RuntimeError(f"submit {i}: {dlu.status_string(st)}")
         srv.drain()
-        comps = {c["id"]: c for c in srv.poll(count + 16)}
+        comps = {c["id"]: c for c in srv.poll(upto + 16)}
         return t0, comps
 
-    out = {"workload": f"{count} requests (the first {count} of the seeded stream; the full "
-                       "configuration has 20000), n log-uniform [64,4096], 1-8 RHS, 30% reuse",
+    out = {"workload": f"{count} requests" + ("" if count == 20000 else
+                                              f" (the first {count} of the seeded stream; the "
+                                              "full configuration has 20000)")
+                       + ", n log-uniform [64,4096], 1-8 RHS, 30% reuse; matrices are windows "
+                         "(lda 4096) of one N(0,1) pool",
            "flops": flops}
     srv = dlu.Server(budget_gib=64.0, mixed=False)
     cutoff = srv.cutoff
-    # capacity of the FP64 service, closed loop over the whole stream
-    t0, comps = run(srv, None)
+    # capacity of the FP64 service, closed loop over the stream's first 5000 requests
+    probe = min(count, 5000)
+    t0, comps = run(srv, None, probe)
     span = max(c["t_done"] for c in comps.values()) - t0
-    capacity = count / span
+    capacity = probe / span
     rate = 0.7 * capacity
     srv.close()
     out.update({"batch_cutoff": cutoff, "closed_loop_requests_per_s": round(capacity, 1),
@@ -273,7 +283,7 @@ def bench_serving(dlu, torch, dev, count, seed=20000):
         worst = 0.0
         for i in range(0, count, max(1, count // 200)):  # a sample, checked on the device
             n = reqs[i]["n"]
-            st, bw = dlu.backward_error_device(n, reqs[i]["nrhs"], a_of(i).data_ptr(), n,
+            st, bw = dlu.backward_error_device(n, reqs[i]["nrhs"], a_of(i), ld,
                                                reqs[i]["x"].data_ptr(), n,
                                                reqs[i]["b"].data_ptr(), n)
             worst = max(worst, float(bw.max()))
@@ -286,10 +296,11 @@ def bench_serving(dlu, torch, dev, count, seed=20000):
             "tflops": round(flops / span / 1e12, 3),
             "batched": sum(1 for c in comps.values() if c["path"] == 0),
             "reused": sum(1 for c in comps.values() if c["reused"]),
+            "graph_replayed": sum(1 for c in comps.values() if c["graph_sweeps"] > 0),
             "all_succeeded": ok, "max_backward_error_sampled": worst}
         log(f"serving mixed={mixed}: p50 {out['mixed' if mixed else 'fp64']['p50_ms']} ms "
             f"p99 {out['mixed' if mixed else 'fp64']['p99_ms']} ms")
-    del mats, reqs
+    del pool, reqs
     torch.cuda.empty_cache()
     return out
 
@@ -311,7 +322,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the delay-stress, mixed-precision and serving measurements")
     ap.add_argument("--mixed-n", type=int, default=16384)
-    ap.add_argument("--serve-requests", type=int, default=1500,
+    ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests of the serving stream measured per precision mode")
     ap.add_argument("--gemm", type=int, nargs=2, metavar=("M", "K"), default=None,
                     help="only time the trailing-update kernel C(M x M) -= A(M x K) B(K x M) "
