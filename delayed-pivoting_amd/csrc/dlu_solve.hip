@@ -39,13 +39,16 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const T *__restrict__ 
 // A solved block travels through xw as tagged exchange words (an LL protocol: 32 payload bits
 // and the launch's 32-bit epoch in each 64-bit word, two words per FP64 value), so a reader
 // spins on the data itself with relaxed loads and no flag, fence or cache invalidation sits on
-// the chain.  Per block: before any wait, wave 0 inverts the four 16 x 16 diagonal sub-blocks
-// (LDS) and loads the off-diagonal sub-blocks of the diagonal block; every wave (16 rows each)
-// subtracts the contribution of each earlier block as its words arrive (MFMA, the A operands of
-// the next block loaded ahead); then wave 0 gathers the 64 rows and solves the diagonal block
-// alone, in registers, and publishes the rows.  Rows at and beyond n are zero before the
-// diagonal solve and stay zero.  Waits are bounded: after spin_limit ticks of the 100 MHz
-// clock a wave sets *err and goes on.
+// the chain.  Per block i, before the wait that closes it: the four waves form the inverse of
+// the 64 x 64 diagonal block D (its 16 x 16 diagonal sub-blocks inverted in LDS, then a block
+// substitution on the identity, 16 columns per wave) and M = D^{-1} (-T_{i,jp}), the coupling to
+// the block jp solved just before; they subtract the contribution of every earlier block as
+// its words arrive (MFMA into four accumulators, the A operands of the next block loaded
+// ahead), and form y = D^{-1} (what is left of the right-hand side).  The chain's critical path
+// per block is then the arrival of block jp's words, x_i = y + M x_jp (16 MFMAs per wave, no
+// exchange between the waves) and the publication of the rows.  Rows at and beyond n are zero
+// in y and M, and so in x_i (the inverse is the identity there).  Waits are bounded: after
+// spin_limit ticks of the 100 MHz clock a wave sets *err and goes on.
 //
 // The launch's ticket base and epoch come from the arguments, or, when clock is not null, from
 // device memory (written by chain_clock_kernel earlier on the stream): a launch recorded in a
@@ -84,6 +87,8 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
   constexpr int WPV = CW::n;
   __shared__ int s_blk;
   __shared__ T s_d[4][16][17];  // diagonal sub-blocks, then their inverses
+  __shared__ T s_m[64][65];     // the block's inverse
+  __shared__ T s_c[64][65];     // M, the coupling to the block solved just before
   __shared__ T s_acc[64][17];   // the block's right-hand sides after all contributions
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int lr = lane & 15, lk = lane >> 4;
@@ -103,8 +108,12 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
   const int last = n - 1;
   const long long t0 = static_cast<long long>(wall_clock64());
 
-  // 1. Diagonal sub-blocks into LDS (entries outside [0, n) read a clamped address, then become
-  // those of the identity), inverted in place by one thread per sub-block column.
+  // 1. The diagonal block's inverse, before any wait.  Its four 16 x 16 diagonal sub-blocks go
+  // to LDS (entries outside [0, n) become those of the identity) and are inverted in place by
+  // one thread per sub-block column; then wave w runs the block substitution over the four
+  // sub-blocks on identity columns [16w, 16w + 16), the off-diagonal sub-blocks as negated A
+  // operands in the k order of an MFMA result tile (k-step s of lane l is row acc_row(l, s) of
+  // the B tile), zero outside [0, n).
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int idx = 256 * j + t;
@@ -137,12 +146,8 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
 #pragma unroll
     for (int r = 0; r < 16; ++r) s_d[sb][r][c] = inv[r];
   }
-  // Wave 0: the six off-diagonal 16 x 16 tiles of the diagonal block as A operands, negated, in
-  // the k order of an MFMA result tile (k-step s of lane l is row acc_row(l, s) of the B tile).
-  // Lower: tile (tr, tc) with tr > tc; upper: tr < tc.  Entries outside [0, n) are never used
-  // with a nonzero B (rows past n are zero), so clamped addresses suffice.
-  T dt[6][4];
-  if (wave == 0) {
+  {
+    T dt[6][4];
     int k6 = 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -153,24 +158,132 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
 #pragma unroll
           for (int s4 = 0; s4 < 4; ++s4) {
             const int cl = r0 + 16 * b + MF::acc_row(lane, s4);
-            dt[k6][s4] = -lu[static_cast<size_t>(row < n ? row : last) +
-                             static_cast<size_t>(cl < n ? cl : last) * lda];
+            const T v = lu[static_cast<size_t>(row < n ? row : last) +
+                           static_cast<size_t>(cl < n ? cl : last) * lda];
+            dt[k6][s4] = (row < n && cl < n) ? -v : T(0);
           }
           ++k6;
         }
+    __syncthreads();  // the inverted sub-blocks
+    acc_t xr[4];
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        xr[tt][r] = (16 * tt + MF::acc_row(lane, r) == 16 * wave + lr) ? T(1) : T(0);
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const int b = UPPER ? 3 - st : st;
+      acc_t dd = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4)
+        dd = MF::mma(s_d[b][lr][MF::acc_row(lane, s4)], xr[b][s4], dd);
+      xr[b] = dd;
+      k6 = 0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (UPPER ? a < c : a > c) {
+            if (c == b) {
+#pragma unroll
+              for (int s4 = 0; s4 < 4; ++s4) xr[a] = MF::mma(dt[k6][s4], xr[b][s4], xr[a]);
+            }
+            ++k6;
+          }
+    }
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s_m[16 * tt + MF::acc_row(lane, r)][16 * wave + lr] = xr[tt][r];
+  }
+  __syncthreads();  // the inverse, every column group
+  const int nj = UPPER ? nblk - 1 - i : i;  // blocks solved before this one
+  const int jp = UPPER ? i + 1 : i - 1;     // the one solved just before
+
+  // 2. M = D^{-1} (-T_{i,jp}), the coupling to the block solved just before: this wave's 16
+  // rows of the inverse (A operand k-step s: column 4s + lk) times T_{i,jp} (B operand k-step
+  // s: row 4s + lk of this block, 16 columns of block jp per tile; entries outside [0, n)
+  // zero), into this wave's rows of s_c.
+  if (nj > 0) {
+#pragma unroll 1
+    for (int ct = 0; ct < 4; ++ct) {
+      acc_t mres = {T(0), T(0), T(0), T(0)};
+      const int cl = 64 * jp + 16 * ct + lr;
+#pragma unroll
+      for (int s4 = 0; s4 < 16; ++s4) {
+        const int row = r0 + 4 * s4 + lk;
+        const T v = lu[static_cast<size_t>(row < n ? row : last) +
+                       static_cast<size_t>(cl < n ? cl : last) * lda];
+        mres = MF::mma(s_m[16 * wave + lr][4 * s4 + lk], (row < n && cl < n) ? -v : T(0), mres);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s_c[16 * wave + MF::acc_row(lane, r)][16 * ct + lr] = mres[r];
+    }
   }
 
-  // 2. This wave's 16 rows of the right-hand side, as an MFMA result tile.
+  // 3. This wave's 16 rows of the right-hand side, as an MFMA result tile; contributions go
+  // to four accumulators by k-step (four independent MFMA chains), summed at the end.
   const int trow = r0 + 16 * wave;
-  acc_t acc;
+  acc_t acc[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
-    acc[r] = x[static_cast<size_t>(trow + MF::acc_row(lane, r)) + static_cast<size_t>(col) * ldx];
+  for (int r = 0; r < 4; ++r) {
+    acc[0][r] =
+        x[static_cast<size_t>(trow + MF::acc_row(lane, r)) + static_cast<size_t>(col) * ldx];
+    acc[1][r] = acc[2][r] = acc[3][r] = T(0);
+  }
+  bool late = false;
+  // Waits until block j's words carry this launch's epoch; its B operand (k-step s4: row
+  // 4 s4 + lk of block j, column lr) lands in lo/hi.  A block's words lie part by part (all low
+  // halves, then all high halves), rows of 16 columns within a part, so one load or store
+  // instruction covers 512 contiguous bytes.  Off the critical path (critical = false) the wait
+  // first spins on the one word that block j's last store instruction writes (row 63, column
+  // 15, last part): a single transaction per wave, so the waiting workgroups do not load every
+  // word through the coherence point that the stores of the block being solved go through.
+  auto wait_block = [&](int j, bool critical, unsigned(&lo)[16], unsigned(&hi)[16]) {
+    const unsigned long long *wj = xw + static_cast<size_t>(j) * (64 * 16 * WPV) + lk * 16 + lr;
+    if (!critical) {
+      const unsigned long long *wlast =
+          xw + static_cast<size_t>(j) * (64 * 16 * WPV) + ((WPV - 1) * 64 + 63) * 16 + 15;
+      while (!late) {
+        const unsigned long long w =
+            __hip_atomic_load(wlast, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (static_cast<unsigned>(w >> 32) == epoch) break;
+        if (static_cast<long long>(wall_clock64()) - t0 > spin_limit) break;  // reported below
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    for (;;) {
+      // Tag mismatches OR-ed together (no short circuit, so every load stays unconditional).
+      unsigned stale = 0u;
+#pragma unroll
+      for (int s4 = 0; s4 < 16; ++s4) {
+        const unsigned long long w0 =
+            __hip_atomic_load(wj + 4 * 16 * s4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        lo[s4] = static_cast<unsigned>(w0);
+        stale |= static_cast<unsigned>(w0 >> 32) ^ epoch;
+        if (WPV == 2) {
+          const unsigned long long w1 = __hip_atomic_load(
+              wj + 64 * 16 + 4 * 16 * s4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          hi[s4] = static_cast<unsigned>(w1);
+          stale |= static_cast<unsigned>(w1 >> 32) ^ epoch;
+        } else {
+          hi[s4] = 0u;
+        }
+      }
+      if (__all(stale == 0u)) break;
+      if (late || static_cast<long long>(wall_clock64()) - t0 > spin_limit) {
+        if (!late && lane == 0) atomicExch(err, 1);
+        late = true;
+        break;
+      }
+    }
+  };
 
-  // 3. Contributions of the blocks solved before this one, as their words arrive.
+  // Contributions of the blocks solved before jp, as their words arrive (the A operands of the
+  // next block loaded ahead).
   const int arow = trow + lr < n ? trow + lr : last;
   const T *arow_p = lu + static_cast<size_t>(arow);
-  const int nj = UPPER ? nblk - 1 - i : i;
   T a_next[16];
   auto load_a = [&](int j, T(&dst)[16]) {
 #pragma unroll
@@ -179,99 +292,67 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
       dst[s4] = -arow_p[static_cast<size_t>(kc < n ? kc : last) * lda];
     }
   };
-  if (nj > 0) load_a(UPPER ? nblk - 1 : 0, a_next);
-  bool late = false;
-  for (int d = 0; d < nj; ++d) {
+  if (nj > 1) load_a(UPPER ? nblk - 1 : 0, a_next);
+  for (int d = 0; d + 1 < nj; ++d) {
     const int j = UPPER ? nblk - 1 - d : d;
     T a_cur[16];
 #pragma unroll
     for (int s4 = 0; s4 < 16; ++s4) a_cur[s4] = a_next[s4];
-    if (d + 1 < nj) load_a(UPPER ? j - 1 : j + 1, a_next);
-    // B operand k-step s4: row 4 s4 + lk of block j, column lr.
-    const unsigned long long *wj =
-        xw + (static_cast<size_t>(64 * j + lk) * 16 + lr) * WPV;
+    if (d + 2 < nj) load_a(UPPER ? j - 1 : j + 1, a_next);
     unsigned lo[16], hi[16];
-    for (;;) {
-      bool fresh = true;
+    wait_block(j, false, lo, hi);
 #pragma unroll
-      for (int s4 = 0; s4 < 16; ++s4) {
-        const unsigned long long w0 = __hip_atomic_load(wj + 4 * 16 * WPV * s4, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT);
-        lo[s4] = static_cast<unsigned>(w0);
-        fresh = fresh && static_cast<unsigned>(w0 >> 32) == epoch;
-        if (WPV == 2) {
-          const unsigned long long w1 = __hip_atomic_load(
-              wj + 4 * 16 * WPV * s4 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          hi[s4] = static_cast<unsigned>(w1);
-          fresh = fresh && static_cast<unsigned>(w1 >> 32) == epoch;
-        } else {
-          hi[s4] = 0u;
-        }
-      }
-      if (__all(fresh)) break;
-      if (late || static_cast<long long>(wall_clock64()) - t0 > spin_limit) {
-        if (!late && lane == 0) atomicExch(err, 1);
-        late = true;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-#pragma unroll
-    for (int s4 = 0; s4 < 16; ++s4) acc = MF::mma(a_cur[s4], CW::join(lo[s4], hi[s4]), acc);
+    for (int s4 = 0; s4 < 16; ++s4)
+      acc[s4 & 3] = MF::mma(a_cur[s4], CW::join(lo[s4], hi[s4]), acc[s4 & 3]);
   }
 
-  // 4. The 64 rows to wave 0 (rows past n zeroed), which solves the diagonal block alone.
+  // 4. y = D^{-1} (right-hand side after those contributions): the 64 rows (past n zeroed) meet
+  // in LDS and each wave multiplies its rows of the inverse by them.
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int rr = 16 * wave + MF::acc_row(lane, r);
-    s_acc[rr][lr] = (r0 + rr < n) ? acc[r] : T(0);
+    const T v = (acc[0][r] + acc[1][r]) + (acc[2][r] + acc[3][r]);
+    s_acc[rr][lr] = (r0 + rr < n) ? v : T(0);
   }
   __syncthreads();
-  if (wave != 0) return;
-  acc_t xr[4];
+  acc_t xp[4];
 #pragma unroll
-  for (int tt = 0; tt < 4; ++tt)
+  for (int q = 0; q < 4; ++q) xp[q] = acc_t{T(0), T(0), T(0), T(0)};
 #pragma unroll
-    for (int r = 0; r < 4; ++r) xr[tt][r] = s_acc[16 * tt + MF::acc_row(lane, r)][lr];
-  // Substitution over the four sub-blocks: X_b := inv(D_bb) X_b, then X_t -= D_tb X_b.  With
-  // k in the result's row order, a result tile is the B operand of the next product as it is.
+  for (int s4 = 0; s4 < 16; ++s4)
+    xp[s4 & 3] =
+        MF::mma(s_m[16 * wave + lr][4 * s4 + lk], s_acc[4 * s4 + lk][lr], xp[s4 & 3]);
+
+  // 5. The chain step: x_i = y + M x_jp as soon as block jp's words arrive.
+  if (nj > 0) {
 #pragma unroll
-  for (int st = 0; st < 4; ++st) {
-    const int b = UPPER ? 3 - st : st;
-    acc_t dd = {T(0), T(0), T(0), T(0)};
+    for (int r = 0; r < 4; ++r) xp[0][r] = (xp[0][r] + xp[1][r]) + (xp[2][r] + xp[3][r]);
+    xp[1] = xp[2] = xp[3] = acc_t{T(0), T(0), T(0), T(0)};
+    T am[16];  // this wave's rows of M (its own rows of s_c), read before the wait
 #pragma unroll
-    for (int s4 = 0; s4 < 4; ++s4) dd = MF::mma(s_d[b][lr][MF::acc_row(lane, s4)], xr[b][s4], dd);
-    xr[b] = dd;
-    int k6 = 0;
+    for (int s4 = 0; s4 < 16; ++s4) am[s4] = s_c[16 * wave + lr][4 * s4 + lk];
+    unsigned lo[16], hi[16];
+    wait_block(jp, true, lo, hi);
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (UPPER ? a < c : a > c) {
-          if (c == b) {
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) xr[a] = MF::mma(dt[k6][s4], xr[b][s4], xr[a]);
-          }
-          ++k6;
-        }
+    for (int s4 = 0; s4 < 16; ++s4)
+      xp[s4 & 3] = MF::mma(am[s4], CW::join(lo[s4], hi[s4]), xp[s4 & 3]);
   }
 
-  // 5. Publish: tagged words for the blocks that wait on this one, plain values for the next
+  // 6. Publish: tagged words for the blocks that wait on this one, plain values for the next
   // kernel.
 #pragma unroll
-  for (int tt = 0; tt < 4; ++tt)
+  for (int r = 0; r < 4; ++r) {
+    const int row = trow + MF::acc_row(lane, r);
+    const T v = (xp[0][r] + xp[1][r]) + (xp[2][r] + xp[3][r]);
+    unsigned long long *w =
+        xw + static_cast<size_t>(i) * (64 * 16 * WPV) + (row - r0) * 16 + lr;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = r0 + 16 * tt + MF::acc_row(lane, r);
-      const T v = xr[tt][r];
-      unsigned long long *w = xw + (static_cast<size_t>(row) * 16 + lr) * WPV;
-#pragma unroll
-      for (int p = 0; p < WPV; ++p)
-        __hip_atomic_store(w + p,
-                           (static_cast<unsigned long long>(epoch) << 32) | CW::part(v, p),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (lr < nrhs) x[static_cast<size_t>(row) + static_cast<size_t>(lr) * ldx] = v;
-    }
+    for (int p = 0; p < WPV; ++p)
+      __hip_atomic_store(w + p * 64 * 16,
+                         (static_cast<unsigned long long>(epoch) << 32) | CW::part(v, p),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lr < nrhs) x[static_cast<size_t>(row) + static_cast<size_t>(lr) * ldx] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
