@@ -321,6 +321,9 @@ def main():
                     help="skip the CPU baseline and the parity check against the oracle")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the delay-stress, mixed-precision and serving measurements")
+    ap.add_argument("--extra", default="sweep,c3,c4,c5",
+                    help="which extra measurements to run (comma-separated of sweep, c3, c4, "
+                         "c5), e.g. c4 alone for a profiled run of the mixed-precision solve")
     ap.add_argument("--mixed-n", type=int, default=16384)
     ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests of the serving stream measured per precision mode")
@@ -453,11 +456,15 @@ def main():
     del orig, work
     torch.cuda.empty_cache()
     c3 = c4 = c5 = sweep = None
-    if not args.no_extra:
+    extra = set() if args.no_extra else {e.strip() for e in args.extra.split(",") if e.strip()}
+    if "sweep" in extra:
         sweep = bench_sweep(dlu, torch, dev, [2048, 4096, 8192, 16384, 32768])
+    if "c3" in extra:
         c3 = bench_delay_stress(dlu, torch, dev, 8192)
         c3["column_variant"] = bench_delay_stress(dlu, torch, dev, 8192, column=True)
+    if "c4" in extra:
         c4 = bench_mixed(dlu, torch, dev, args.mixed_n, 16)
+    if "c5" in extra:
         c5 = bench_serving(dlu, torch, dev, args.serve_requests)
 
     out = {

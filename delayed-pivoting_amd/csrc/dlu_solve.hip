@@ -356,46 +356,64 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
 }
 
 // ---------------------------------------------------------------------------------------------
-// R = B - A X, FP64, for a 16 x 16 tile of R per workgroup.  The four waves take interleaved
-// k-quarters of the product on v_mfma_f64_16x16x4_f64 and combine their partial tiles in LDS in
-// a fixed order, so the result does not depend on scheduling.
+// R = B - A X, FP64, for 64 rows x 16 columns of R per workgroup.  The product reads A once
+// (it is bound by HBM), so the workgroup takes 64 rows, whose four 16-row MFMA tiles cover 512
+// contiguous bytes of each column of A, and each X operand feeds the four tiles.  The eight
+// waves take interleaved k-steps of v_mfma_f64_16x16x4_f64 and their partial tiles are summed
+// in LDS in a fixed order, so the result does not depend on scheduling.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void residual_kernel(const double *__restrict__ a, int lda,
-                                                      const double *__restrict__ xm, int ldx,
-                                                      const double *__restrict__ b, int ldb,
-                                                      double *__restrict__ r, int ldr, int n,
-                                                      int nrhs) {
+constexpr int kResRows = 64;
+constexpr int kResWaves = 8;
+__global__ __launch_bounds__(64 * kResWaves) void residual_kernel(
+    const double *__restrict__ a, int lda, const double *__restrict__ xm, int ldx,
+    const double *__restrict__ b, int ldb, double *__restrict__ r, int ldr, int n, int nrhs) {
   typedef Mfma<double> MF;
-  __shared__ double part[4][64][4];
+  __shared__ double part[kResWaves][4][64][4];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
+  const int r0 = blockIdx.x * kResRows, c0 = blockIdx.y * 16;
   const int lr = lane & 15, lk = lane >> 4;
-  const int row = r0 + lr, col = c0 + lr;
-  const bool row_ok = row < n, col_ok = col < nrhs;
-  const double *ap = a + static_cast<size_t>(row_ok ? row : 0);
+  const int col = c0 + lr;
+  const bool col_ok = col < nrhs;
   const double *xp = xm + static_cast<size_t>(col_ok ? col : 0) * ldx;
-  MF::acc_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-  for (int k = 4 * wave; k < n; k += 16) {
+  const double *ap[4];
+  bool rok[4];
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt) {
+    const int row = r0 + 16 * tt + lr;
+    rok[tt] = row < n;
+    ap[tt] = a + static_cast<size_t>(rok[tt] ? row : 0);
+  }
+  MF::acc_t acc[4];
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt) acc[tt] = MF::acc_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int k = 4 * wave; k < n; k += 4 * kResWaves) {
     const int kk = k + lk;
     const bool ok = kk < n;
-    const double av = (row_ok && ok) ? ap[static_cast<size_t>(kk) * lda] : 0.0;
     const double xv = (col_ok && ok) ? xp[kk] : 0.0;
-    acc = MF::mma(av, xv, acc);
+    double av[4];
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt)
+      av[tt] = (rok[tt] && ok) ? ap[tt][static_cast<size_t>(kk) * lda] : 0.0;
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) acc[tt] = MF::mma(av[tt], xv, acc[tt]);
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) part[wave][lane][q] = acc[q];
+  for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) part[wave][tt][lane][q] = acc[tt][q];
   __syncthreads();
-  if (wave == 0) {
+  if (t < 256) {  // one output tile per wave of the first four, the eight partials in order
+    const int tt = t >> 6, oc = c0 + (lane & 15);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const double s = ((part[0][lane][q] + part[1][lane][q]) + part[2][lane][q]) +
-                       part[3][lane][q];
-      const int orow = r0 + MF::acc_row(lane, q);
-      if (orow < n && col_ok) {
-        const size_t ob = static_cast<size_t>(orow) + static_cast<size_t>(col) * ldb;
-        r[static_cast<size_t>(orow) + static_cast<size_t>(col) * ldr] = b[ob] - s;
-      }
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < kResWaves; ++w) s += part[w][tt][lane][q];
+      const int orow = r0 + 16 * tt + MF::acc_row(lane, q);
+      if (orow < n && oc < nrhs)
+        r[static_cast<size_t>(orow) + static_cast<size_t>(oc) * ldr] =
+            b[static_cast<size_t>(orow) + static_cast<size_t>(oc) * ldb] - s;
     }
   }
 }
@@ -526,8 +544,8 @@ hipError_t launch_chain_clock(ChainClocks *st, int nslots, int blocks, unsigned 
 hipError_t launch_residual(const double *a, int lda, const double *x, int ldx, const double *b,
                            int ldb, double *r, int ldr, int n, int nrhs, hipStream_t s) {
   if (n <= 0 || nrhs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(residual_kernel, dim3((n + 15) / 16, (nrhs + 15) / 16), dim3(256), 0, s, a,
-                     lda, x, ldx, b, ldb, r, ldr, n, nrhs);
+  hipLaunchKernelGGL(residual_kernel, dim3((n + kResRows - 1) / kResRows, (nrhs + 15) / 16),
+                     dim3(64 * kResWaves), 0, s, a, lda, x, ldx, b, ldb, r, ldr, n, nrhs);
   return hipGetLastError();
 }
 
