@@ -358,12 +358,13 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(
 // ---------------------------------------------------------------------------------------------
 // R = B - A X, FP64, for 64 rows x 16 columns of R per workgroup.  The product reads A once
 // (it is bound by HBM), so the workgroup takes 64 rows, whose four 16-row MFMA tiles cover 512
-// contiguous bytes of each column of A, and each X operand feeds the four tiles.  The eight
-// waves take interleaved k-steps of v_mfma_f64_16x16x4_f64 and their partial tiles are summed
-// in LDS in a fixed order, so the result does not depend on scheduling.
+// contiguous bytes of each column of A, and each X operand feeds the four tiles.  Sixteen
+// waves take interleaved k-steps of v_mfma_f64_16x16x4_f64 (measured at n = 16384: 4 waves
+// 2.5 TB/s, 8 waves 4.5, 16 waves 5.7) and their partial tiles are summed in LDS in a fixed
+// order, so the result does not depend on scheduling.
 // ---------------------------------------------------------------------------------------------
 constexpr int kResRows = 64;
-constexpr int kResWaves = 8;
+constexpr int kResWaves = 16;
 __global__ __launch_bounds__(64 * kResWaves) void residual_kernel(
     const double *__restrict__ a, int lda, const double *__restrict__ xm, int ldx,
     const double *__restrict__ b, int ldb, double *__restrict__ r, int ldr, int n, int nrhs) {
@@ -386,7 +387,7 @@ __global__ __launch_bounds__(64 * kResWaves) void residual_kernel(
   MF::acc_t acc[4];
 #pragma unroll
   for (int tt = 0; tt < 4; ++tt) acc[tt] = MF::acc_t{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
+#pragma unroll 2
   for (int k = 4 * wave; k < n; k += 4 * kResWaves) {
     const int kk = k + lk;
     const bool ok = kk < n;
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(64 * kResWaves) void residual_kernel(
 #pragma unroll
     for (int q = 0; q < 4; ++q) part[wave][tt][lane][q] = acc[tt][q];
   __syncthreads();
-  if (t < 256) {  // one output tile per wave of the first four, the eight partials in order
+  if (t < 256) {  // one output tile per wave of the first four, the partials in order
     const int tt = t >> 6, oc = c0 + (lane & 15);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
