@@ -74,6 +74,23 @@ double dlu_oracle_backward_error(int m, int n, const double *original, const dou
                                  int lda, const int *ipiv, const int *jpiv, int eliminated,
                                  int delayed);
 
+/* Delayed Bunch-Kaufman LDL^T of the symmetric n x n matrix held in the lower triangle of `a`
+ * (the upper triangle is neither read nor written): P^T A P = L D L^T with 1x1 and 2x2 blocks in
+ * D, nodes that take neither deferred to the front of a delayed suffix [info, n) whose Schur
+ * complement stays in place.  perm[pos] = original index of the node at pos; piv_size[pos] = 1
+ * (1x1), 2 (first column of a 2x2), 0 (its second column, or delayed).  info = -1 on invalid
+ * arguments, -2 when the BLAS lacks a routine.
+ *   dlu_oracle_delayed_sytrf          delayed-bunch-kaufman/delayed_sytrf.cpp:177-282 (unblocked)
+ *   dlu_oracle_delayed_sytrf_blocked  delayed-bunch-kaufman/delayed_sytrf_blocked.cpp:168-358
+ *                                     (panels of at most nb columns; the reference uses 64)
+ *   dlu_oracle_sytrf_unblocked_ld     an independent unblocked restatement in long double with
+ *                                     no BLAS (pins the two above) */
+void dlu_oracle_delayed_sytrf(int n, double *a, int lda, int *perm, int *piv_size, int *info);
+void dlu_oracle_delayed_sytrf_blocked(int n, double *a, int lda, int *perm, int *piv_size,
+                                      int *info, int nb);
+void dlu_oracle_sytrf_unblocked_ld(int n, double *a, int lda, int *perm, int *piv_size,
+                                   int *info);
+
 #ifdef __cplusplus
 }
 #endif

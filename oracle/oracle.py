@@ -53,6 +53,10 @@ def load_build(path: str):
     for name in ("dlu_oracle_delayed_getrf", "dlu_oracle_delayed_getrf_column"):
         getattr(L, name).argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip, ip, ip,
                                      ctypes.c_int, dp, ip]
+    sy = [ctypes.c_int, dp, ctypes.c_int, ip, ip, ip]
+    L.dlu_oracle_delayed_sytrf.argtypes = sy
+    L.dlu_oracle_delayed_sytrf_blocked.argtypes = sy + [ctypes.c_int]
+    L.dlu_oracle_sytrf_unblocked_ld.argtypes = sy
     for path_, prefix in _blas_candidates():
         if L.dlu_oracle_init(path_.encode(), prefix.encode()) == 0:
             return L
@@ -103,6 +107,10 @@ def lib():
     L.dlu_oracle_backward_error.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int,
                                             ip, ip, ctypes.c_int, ctypes.c_int]
     L.dlu_oracle_backward_error.restype = ctypes.c_double
+    sy = [ctypes.c_int, dp, ctypes.c_int, ip, ip, ip]
+    L.dlu_oracle_delayed_sytrf.argtypes = sy
+    L.dlu_oracle_delayed_sytrf_blocked.argtypes = sy + [ctypes.c_int]
+    L.dlu_oracle_sytrf_unblocked_ld.argtypes = sy
     errors = []
     for path, prefix in _blas_candidates():
         rc = L.dlu_oracle_init(path.encode(), prefix.encode())
@@ -213,6 +221,70 @@ def backward_error(original: np.ndarray, factor: np.ndarray, ipiv, jpiv: Optiona
     jp = np.ascontiguousarray(jpiv, dtype=np.int32) if jpiv is not None else None
     return lib().dlu_oracle_backward_error(m, n, _dp(o), _dp(f), max(1, m), _ip(ip), _ip(jp),
                                            int(info), 1 if delayed else 0)
+
+
+def delayed_sytrf(a: np.ndarray, blocked: bool = True, nb: int = 64, ld: bool = False,
+                  library=None):
+    """Delayed Bunch-Kaufman LDL^T of the symmetric matrix in the lower triangle of `a` (a copy):
+    blocked (delayed_sytrf_blocked.cpp, panels of nb), unblocked (delayed_sytrf.cpp), or the
+    independent long-double restatement (ld=True).  Returns (factor, perm, piv_size, info)."""
+    f = _fortran(a)
+    n = f.shape[0]
+    perm = np.zeros(max(n, 1), dtype=np.int32)
+    piv = np.zeros(max(n, 1), dtype=np.int32)
+    info = ctypes.c_int(-1)
+    L = library or lib()
+    if ld:
+        L.dlu_oracle_sytrf_unblocked_ld(n, _dp(f), max(1, n), _ip(perm), _ip(piv),
+                                        ctypes.byref(info))
+    elif blocked:
+        L.dlu_oracle_delayed_sytrf_blocked(n, _dp(f), max(1, n), _ip(perm), _ip(piv),
+                                           ctypes.byref(info), nb)
+    else:
+        L.dlu_oracle_delayed_sytrf(n, _dp(f), max(1, n), _ip(perm), _ip(piv), ctypes.byref(info))
+    if info.value < -1:
+        raise RuntimeError("oracle: the BLAS lacks a routine of the symmetric factorization")
+    return f, perm[:n], piv[:n], info.value
+
+
+def sytrf_parts(factor: np.ndarray, piv_size, info: int):
+    """Unit lower L and block-diagonal D of a delayed LDL^T factor (delayed-bunch-kaufman/
+    example.cpp:47-148): L's column of a 1x1 pivot holds its multipliers below the diagonal, a
+    2x2 pivot's two columns hold theirs below the block; D has the 1x1 values, the symmetric 2x2
+    blocks, and the delayed part's Schur complement as its last block (rows/columns [info, n),
+    read symmetrically from the lower triangle)."""
+    f = np.asarray(factor, dtype=np.float64)
+    n = f.shape[0]
+    L = np.eye(n)
+    D = np.zeros((n, n))
+    k = 0
+    while k < info:
+        if piv_size[k] == 2:
+            D[k, k], D[k + 1, k + 1] = f[k, k], f[k + 1, k + 1]
+            D[k + 1, k] = D[k, k + 1] = f[k + 1, k]
+            L[k + 2:, k] = f[k + 2:, k]
+            L[k + 2:, k + 1] = f[k + 2:, k + 1]
+            k += 2
+        else:
+            D[k, k] = f[k, k]
+            L[k + 1:, k] = f[k + 1:, k]
+            k += 1
+    s = np.tril(f[info:, info:])
+    D[info:, info:] = s + np.tril(s, -1).T
+    return L, D
+
+
+def sytrf_reconstruction_error(original: np.ndarray, factor: np.ndarray, perm, piv_size,
+                               info: int) -> float:
+    """max |P^T A P - L D L^T| relative to max |A| (A read symmetrically from its lower
+    triangle), as delayed-bunch-kaufman/example.cpp:97-148 checks it."""
+    o = np.tril(np.asarray(original, dtype=np.float64))
+    a = o + np.tril(o, -1).T
+    L, D = sytrf_parts(factor, piv_size, info)
+    p = np.asarray(perm)
+    err = np.abs(a[np.ix_(p, p)] - L @ D @ L.T).max()
+    scale = np.abs(a).max()
+    return float(err / scale) if scale > 0 else float(err)
 
 
 def replay(piv, count: int, size: int) -> np.ndarray:

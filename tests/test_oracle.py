@@ -309,3 +309,137 @@ def test_contribution_indices_and_parent_positions():
     assert c.tolist() == [12, 10, 20, 21]  # cols after the swap: 13, 11, 12, 10
     ids, pos = dlu.parent_positions([20, 21, 30], [10, 13, 20, 21])
     assert ids.tolist() == [20, 21, 30, 10, 13] and pos.tolist() == [3, 4, 0, 1]
+
+
+# ---- delayed Bunch-Kaufman LDL^T (delayed-bunch-kaufman/) ------------------------------------
+
+def _sym(n, seed, zero_nodes=()):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, n))
+    s = np.asfortranarray(x + x.T)
+    z = list(zero_nodes)
+    s[z, :] = 0.0
+    s[:, z] = 0.0
+    return s
+
+
+def _lapack_dsytrf_pivots(s):
+    """LAPACK DSYTRF (lower)'s interchanges in the delayed format: perm by applying each step's
+    symmetric interchange in order, piv_size 1 / 2 0 (delayed-bunch-kaufman/lapack_dsytrf.cpp
+    converts the same way through DSYCONV)."""
+    import scipy.linalg.lapack as la
+    n = s.shape[0]
+    _, ipiv, info = la.dsytrf(s, lower=1)
+    perm, piv, k = np.arange(n), np.zeros(n, dtype=int), 0
+    while k < n:
+        if ipiv[k] > 0:
+            p = ipiv[k] - 1
+            perm[[k, p]] = perm[[p, k]]
+            piv[k] = 1
+            k += 1
+        else:
+            p = -ipiv[k] - 1
+            perm[[k + 1, p]] = perm[[p, k + 1]]
+            piv[k], piv[k + 1] = 2, 0
+            k += 2
+    return perm, piv, info
+
+
+def _example_cpp_sytrf():
+    """delayed-bunch-kaufman/example.cpp:248-266: n = 4096, A(3,2) = A(2,3) = 2 (a 2x2 indefinite
+    pivot), A(4,4) .. A(9,9) = 6 .. 1 (six 1x1 pivots), every other entry zero."""
+    n = 4096
+    a = np.zeros((n, n), order="F")
+    a[3, 2] = a[2, 3] = 2.0
+    for k in range(6):
+        a[k + 4, k + 4] = 6.0 - k
+    return a
+
+
+@pytest.mark.parametrize("mode", ["blocked", "unblocked", "ld"])
+def test_sytrf_known_answer_example_cpp(mode):
+    """Derived by hand: node 0 is zero and is deferred to position 4095; each node then brought
+    to position 0 (4095, 4094, ..., 10) is zero and deferred to the front of the suffix, until
+    node 9 arrives (1x1, d = 1).  Node 1 (zero) is deferred to position 8 and node 8 takes its
+    place (1x1, d = 2); nodes 2, 3 have zero diagonals and A(3,2) = 2 (colmax = rowmax = 2, the
+    2x2 block [[0, 2], [2, 0]] is stable); nodes 4..7 are 1x1 (6, 5, 4, 3).  info = 8, as
+    example.cpp's check expects (rank 8)."""
+    a = _example_cpp_sytrf()
+    f, perm, piv, info = oracle.delayed_sytrf(a, blocked=mode == "blocked", ld=mode == "ld")
+    n = a.shape[0]
+    assert info == 8
+    assert perm.tolist() == [9, 8, 2, 3, 4, 5, 6, 7, 1] + list(range(10, n)) + [0]
+    assert piv.tolist() == [1, 1, 2, 0, 1, 1, 1, 1] + [0] * (n - 8)
+    assert [f[0, 0], f[1, 1], f[2, 2], f[3, 2], f[3, 3]] == [1.0, 2.0, 0.0, 2.0, 0.0]
+    assert [f[k, k] for k in range(4, 8)] == [6.0, 5.0, 4.0, 3.0]
+    assert np.count_nonzero(np.tril(f, -1)[:, :8]) == 1  # only the 2x2 block's A(3,2): L = 0
+    assert not f[8:, 8:].any()
+    assert oracle.sytrf_reconstruction_error(a, f, perm, piv, info) == 0.0
+
+
+@pytest.mark.parametrize("n,seed", [(100, 1), (300, 2), (700, 3), (1500, 4)])
+def test_sytrf_matches_lapack_dsytrf(n, seed):
+    """Without delays the rules are LAPACK's Bunch-Kaufman (alpha = (1 + sqrt 17) / 8, the same
+    three tests), so the interchanges and block sizes must equal DSYTRF's."""
+    s = _sym(n, seed)
+    perm_l, piv_l, info_l = _lapack_dsytrf_pivots(s)
+    f, perm, piv, info = oracle.delayed_sytrf(s)
+    assert info_l == 0 and info == n
+    assert np.array_equal(perm, perm_l) and np.array_equal(piv, piv_l)
+    assert (piv == 2).sum() > 0  # the case exercises 2x2 pivots
+    assert oracle.sytrf_reconstruction_error(s, f, perm, piv, info) < 1e-13
+
+
+@pytest.mark.parametrize("n,seed,nz", [(60, 5, 6), (200, 6, 20), (333, 7, 40), (150, 8, 0)])
+def test_sytrf_restatements_agree(n, seed, nz):
+    """The blocked restatement at panel widths 64, 16 and 2, the unblocked one, and the
+    independent long-double one with no BLAS choose the same pivots and the same delayed set;
+    every factorization reconstructs P^T A P."""
+    z = np.random.default_rng(seed).choice(n, size=nz, replace=False)
+    s = _sym(n, seed, z)
+    ref = oracle.delayed_sytrf(s, ld=True)
+    for kw in ({"nb": 64}, {"nb": 16}, {"nb": 2}, {"blocked": False}):
+        f, perm, piv, info = oracle.delayed_sytrf(s, **kw)
+        assert info == ref[3] == n - nz, kw
+        assert np.array_equal(perm, ref[1]) and np.array_equal(piv, ref[2]), kw
+        assert oracle.sytrf_reconstruction_error(s, f, perm, piv, info) < 1e-13, kw
+    assert sorted(ref[1][ref[3]:].tolist()) == sorted(z.tolist())  # the delayed set
+
+
+def test_sytrf_leaves_the_upper_triangle_alone():
+    s = _sym(90, 9)
+    s[np.triu_indices(90, 1)] = 7.25  # not the symmetric values: never read, never written
+    f, perm, piv, info = oracle.delayed_sytrf(s)
+    assert np.all(f[np.triu_indices(90, 2)] == 7.25)
+    g = oracle.delayed_sytrf(np.tril(s) + np.tril(s, -1).T)
+    assert np.array_equal(perm, g[1]) and np.array_equal(piv, g[2])
+
+
+def test_sytrf_unstable_2x2_is_deferred():
+    """A candidate pair whose 2x2 block is singular to working precision ([[1, 2], [2, 4]],
+    with no diagonal large enough for a 1x1 pivot) is not accepted: node 0 goes to the delayed
+    suffix."""
+    a = np.zeros((3, 3), order="F")
+    a[0, 0], a[1, 0], a[1, 1] = 1.0, 2.0, 4.0
+    a[2, 2] = 5.0
+    for kw in ({}, {"blocked": False}, {"ld": True}):
+        f, perm, piv, info = oracle.delayed_sytrf(a, **kw)
+        assert info == 2 and perm[2] == 0, (kw, perm, info)
+
+
+def test_sytrf_planted_mistake_is_caught(tmp_path):
+    """A mutant built with DLU_ORACLE_MUTANT_BK_NOSWAP takes the candidate's diagonal as a 1x1
+    pivot without bringing the candidate to k: the long-double pin must disagree with it on an
+    input where that rule fires, and agree with the real build."""
+    path = oracle.build(force=True, out=str(tmp_path / "libdlu_oracle_bk_mutant.so"),
+                        defines=["DLU_ORACLE_MUTANT_BK_NOSWAP"])
+    mutant = oracle.load_build(path)
+    a = np.zeros((3, 3), order="F")
+    # |a00| < alpha |a10|, |a11| >= alpha * rowmax; node 0's Schur complement after node 1,
+    # 0.2 - 1 / 10, does not cancel
+    a[0, 0], a[1, 0], a[1, 1], a[2, 2] = 0.2, 1.0, 10.0, 3.0
+    ref = oracle.delayed_sytrf(a, ld=True)
+    real = oracle.delayed_sytrf(a)
+    bad = oracle.delayed_sytrf(a, library=mutant)
+    assert ref[1].tolist() == real[1].tolist() and ref[1][0] == 1
+    assert bad[1].tolist() != ref[1].tolist()
