@@ -193,6 +193,13 @@ def lib():
     L.dlu_extend_add_device.restype = ctypes.c_int
     L.dlu_shutdown.argtypes = []
     L.dlu_shutdown.restype = None
+    sy = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip]
+    L.dlu_sytrf_device.argtypes = sy + [ctypes.c_void_p]
+    L.dlu_sytrf_device.restype = ctypes.c_int
+    L.dlu_sytrf_host.argtypes = sy + [ctypes.c_void_p]
+    L.dlu_sytrf_host.restype = ctypes.c_int
+    L.dlu_sytrf_device_nb.argtypes = sy + [ctypes.c_int, ctypes.c_void_p]
+    L.dlu_sytrf_device_nb.restype = ctypes.c_int
     _lib = L
     # Free libdlu's cached device state while the HIP runtime is still up (atexit handlers run
     # before the interpreter and the runtime's own teardown).
@@ -328,6 +335,18 @@ def sgetrf_device(m: int, n: int, d_a: int, lda: int, stream=None,
                                 marg.ctypes.data_as(_dp) if margins else None)
     return (Status(st), ipiv[:p], jpiv[:p], info.value, rep.as_dict(),
             marg[:p] if margins else None)
+
+
+def sytrf_device(n: int, d_a: int, lda: int, stream=None, nb: int = 64):
+    """Delayed Bunch-Kaufman LDL^T of the symmetric device matrix (lower triangle) at ``d_a``:
+    (status, perm, piv_size, info)."""
+    L = lib()
+    perm = np.zeros(max(n, 1), dtype=np.int32)
+    piv = np.zeros(max(n, 1), dtype=np.int32)
+    info = ctypes.c_int(-7)
+    st = L.dlu_sytrf_device_nb(n, d_a, lda, perm.ctypes.data_as(_ip), piv.ctypes.data_as(_ip),
+                               ctypes.byref(info), nb, _stream_ptr(stream))
+    return Status(st), perm[:n], piv[:n], info.value
 
 
 def getrs_device(n: int, nrhs: int, d_lu: int, lda: int, ipiv, jpiv, info: int, d_b: int,

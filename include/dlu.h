@@ -314,6 +314,27 @@ dlu_status dlu_server_drain(dlu_server *server);
 double dlu_server_now(const dlu_server *server);
 void dlu_server_destroy(dlu_server *server);
 
+/* ---- Delayed Bunch-Kaufman LDL^T (delayed-bunch-kaufman/; CPU-only in the reference) ------- */
+
+/* delayed_sytrf_blocked (delayed-bunch-kaufman/delayed_sytrf_blocked.hpp:16, .cpp:168-358) on the
+ * GPU.  The symmetric n x n matrix is the lower triangle of the column-major d_a (the upper
+ * triangle is neither read nor written).  On return P^T A P = L D L^T: D's 1x1 and 2x2 blocks on
+ * the diagonal (a 2x2 block's off-diagonal at (k+1, k)), L's multipliers below them (unit
+ * diagonal implied), and nodes that took neither a stable 1x1 nor a stable 2x2 pivot in the
+ * delayed suffix [info, n), whose Schur complement stays in place (lower triangle).  perm[pos]
+ * (host) = the original index of the node at pos; piv_size[pos] (host) = 1 for a 1x1 pivot, 2 for
+ * the first column of a 2x2 pivot, 0 for its second column or a delayed node.  info (host) = the
+ * number of eliminated nodes.  Panels of 64 columns, as the reference's.  The call synchronizes
+ * `stream` before it returns.  The _host form copies lda*n doubles in and out. */
+dlu_status dlu_sytrf_device(int n, double *d_a, int lda, int *perm, int *piv_size, int *info,
+                            void *stream);
+dlu_status dlu_sytrf_host(int n, double *a, int lda, int *perm, int *piv_size, int *info,
+                          void *stream);
+/* No counterpart in the reference: dlu_sytrf_device with panels of nb columns (2 <= nb <= 64),
+ * for tests of the panel boundaries. */
+dlu_status dlu_sytrf_device_nb(int n, double *d_a, int lda, int *perm, int *piv_size, int *info,
+                               int nb, void *stream);
+
 /* ---- Frontal-matrix steps (the design note's multifrontal use of the delayed suffix,
  * doc/（非）对称稠密矩阵的延迟主元LU方法.md:20-440; the reference implements only Step 1, as
  * delayed_getrf) ------------------------------------------------------------------------------ */
