@@ -176,6 +176,48 @@ def bench_mixed(dlu, torch, dev, n, nrhs):
     return out
 
 
+def bench_sytrf(dlu, torch, dev, sizes, cpu_n, threads):
+    """The delayed Bunch-Kaufman LDL^T (delayed_sytrf_blocked) on symmetric N(0,1) matrices
+    A = X + X^T: GPU time per n (best of two after a warm-up), n^3/3 flops counted; the oracle
+    (C restatement, MKL/OpenBLAS on the host) on the cpu_n matrix, with the permutations and
+    block sizes compared."""
+    import numpy as np
+    out = {"workload": "delayed Bunch-Kaufman LDL^T, A = X + X^T, X N(0,1) seeded, device "
+                       "interface, panels of 64", "flops_counted": "n^3/3", "gpu": []}
+    for n in sizes:
+        log(f"sytrf n={n}")
+        g = torch.Generator(device=dev)
+        g.manual_seed(n + 7)
+        x = torch.randn((n, n), dtype=torch.float64, device=dev, generator=g)
+        orig = (x + x.T).contiguous()  # symmetric: storage rows = columns
+        del x
+        work = torch.empty_like(orig)
+
+        def run():
+            work.copy_(orig)
+            return dlu.sytrf_device(n, work.data_ptr(), n)
+
+        t, (st, perm, piv, info) = _timed(torch, run)
+        out["gpu"].append({"n": n, "ms": round(t * 1e3, 2),
+                           "gflops": round(n ** 3 / 3.0 / t / 1e9, 1), "info": info,
+                           "two_by_two": int((piv == 2).sum()), "status": int(st)})
+        if n == cpu_n and threads > 0:
+            oracle = _load("dlu_oracle_py", os.path.join(ROOT, "oracle", "oracle.py"))
+            oracle.set_threads(threads)
+            a = np.asfortranarray(orig.cpu().numpy())
+            t0 = time.perf_counter()
+            _, pc, qc, ic = oracle.delayed_sytrf(a)
+            tc = time.perf_counter() - t0
+            out["cpu_oracle"] = {"n": n, "ms": round(tc * 1e3, 1), "threads": threads,
+                                 "gflops": round(n ** 3 / 3.0 / tc / 1e9, 1),
+                                 "same_perm_and_blocks": bool(np.array_equal(pc, perm) and
+                                                              np.array_equal(qc, piv) and
+                                                              ic == info)}
+        del orig, work
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_sweep(dlu, torch, dev, sizes):
     """Plain FP64 delayed LU over the size sweep of the headline configuration: best of two
     factorizations after one warm-up, N(0,1) seeded per n, device interface."""
@@ -326,9 +368,10 @@ def main():
                     help="skip the CPU baseline and the parity check against the oracle")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the delay-stress, mixed-precision and serving measurements")
-    ap.add_argument("--extra", default="sweep,c3,c4,c5",
+    ap.add_argument("--extra", default="sweep,c3,c4,c5,sytrf",
                     help="which extra measurements to run (comma-separated of sweep, c3, c4, "
-                         "c5), e.g. c4 alone for a profiled run of the mixed-precision solve")
+                         "c5, sytrf), e.g. c4 alone for a profiled run of the mixed-precision "
+                         "solve")
     ap.add_argument("--mixed-n", type=int, default=16384)
     ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests of the serving stream measured per precision mode")
@@ -460,7 +503,7 @@ def main():
 
     del orig, work
     torch.cuda.empty_cache()
-    c3 = c4 = c5 = sweep = None
+    c3 = c4 = c5 = sweep = sytrf = None
     extra = set() if args.no_extra else {e.strip() for e in args.extra.split(",") if e.strip()}
     if "sweep" in extra:
         sweep = bench_sweep(dlu, torch, dev, [2048, 4096, 8192, 16384, 32768])
@@ -471,6 +514,11 @@ def main():
         c4 = bench_mixed(dlu, torch, dev, args.mixed_n, 16)
     if "c5" in extra:
         c5 = bench_serving(dlu, torch, dev, args.serve_requests)
+    if "sytrf" in extra:
+        sytrf = bench_sytrf(dlu, torch, dev, [4096, 8192], 4096,
+                            0 if args.no_cpu else (args.cpu_threads or int(
+                                os.environ.get("OMP_NUM_THREADS", "0") or 0) or
+                                min(16, os.cpu_count() or 1)))
 
     out = {
         "metric": "FP64 LU TFLOP/s @n=16384 (% peak); FP32+IR speedup @bwd err<=1e-13; p99 ms",
@@ -486,6 +534,7 @@ def main():
         "pivot_report": {k: first[2][k] for k in ("fallbacks", "near_ties", "exact_ties",
                                                   "min_margin", "panels")} if first[2] else None,
         "c2_sweep": sweep, "c3_delay_stress": c3, "c4_mixed_precision": c4, "c5_serving": c5,
+        "f4_sytrf": sytrf,
     }
     print(json.dumps(out), flush=True)
     return 0
