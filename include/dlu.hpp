@@ -50,6 +50,27 @@ inline Status delayed_getrf_hybrid_host(int m, int n, double *a, int lda, int *i
   return delayed_getrf_host(m, n, a, lda, ipiv, jpiv, info, stream);
 }
 
+// delayed_lu/delayed_dgetrf_column.hpp:19 (the column-delay variant, on the GPU)
+inline Status delayed_getrf_column_device(int m, int n, double *d_a, int lda, int *ipiv,
+                                          int *jpiv, int &info, hipStream_t stream = nullptr) {
+  return static_cast<Status>(dlu_getrf_column_device(m, n, d_a, lda, ipiv, jpiv, &info, stream));
+}
+inline Status delayed_getrf_column_host(int m, int n, double *a, int lda, int *ipiv, int *jpiv,
+                                        int &info, hipStream_t stream = nullptr) {
+  return static_cast<Status>(dlu_getrf_column_host(m, n, a, lda, ipiv, jpiv, &info, stream));
+}
+
+// delayed-bunch-kaufman/delayed_sytrf_blocked.hpp:16 (the delayed Bunch-Kaufman LDL^T, on the
+// GPU)
+inline Status delayed_sytrf_blocked_device(int n, double *d_a, int lda, int *perm, int *piv_size,
+                                           int &info, hipStream_t stream = nullptr) {
+  return static_cast<Status>(dlu_sytrf_device(n, d_a, lda, perm, piv_size, &info, stream));
+}
+inline Status delayed_sytrf_blocked_host(int n, double *a, int lda, int *perm, int *piv_size,
+                                         int &info, hipStream_t stream = nullptr) {
+  return static_cast<Status>(dlu_sytrf_host(n, a, lda, perm, piv_size, &info, stream));
+}
+
 // delayed_dgetrf_cuda.hpp:63
 inline const char *status_string(Status s) {
   return dlu_status_string(static_cast<dlu_status>(s));
