@@ -558,7 +558,10 @@ dlu_server *dlu_server_create(const dlu_server_config *config) {
   s->cutoff = s->cfg.batch_cutoff > 0    ? std::min(s->cfg.batch_cutoff, DLU_BATCH_MAX_N)
               : s->cfg.batch_cutoff == 0 ? std::min(dlu::kTunedBatchCutoff, DLU_BATCH_MAX_N)
                                          : s->tune_cutoff();
-  const int workers = s->cfg.large_workers > 0 ? std::min(s->cfg.large_workers, 4) : 3;
+  // Four full workers by default (one per factorization workspace of the device): measured on
+  // the 20000-request stream, four against three served 11 % more requests per second at a
+  // lower p50 and p99 in both precision modes.
+  const int workers = s->cfg.large_workers > 0 ? std::min(s->cfg.large_workers, 4) : 4;
   for (int i = 0; i < workers; ++i) {
     hipStream_t st = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) break;

@@ -269,7 +269,7 @@ typedef struct dlu_server_config {
   double tol;            /* refinement target backward error (default 1e-13) */
   int max_batch;         /* most requests per batched launch (default 1024) */
   int large_workers;     /* requests above the cutoff factored at once, each on its own stream
-                            (default 3, at most 4) */
+                            (default 4, at most 4) */
   int eager_sweeps;      /* 0 (default): a later right-hand side of a resident factor (above
                             the cutoff, nrhs <= 128) replays the factor's refinement sweep from a
                             HIP graph recorded at its first reuse; 1: every sweep is launched
