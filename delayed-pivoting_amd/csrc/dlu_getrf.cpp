@@ -269,6 +269,9 @@ struct Run {
   hipStream_t s;  // the caller's stream
   long long spin;
   bool profile = false;
+  // Per-step top-2 margins and the tie statistics: tracked only when the caller asks for the
+  // margins (the reference reports none, and the bookkeeping sits on the panel step).
+  bool track_margins = false;
   // Everything in order on the caller's stream, with no events and no pinned host memory (the
   // allocation-free entry point); look-ahead, pairs and the panel-stream tail are off then.
   bool single = false;
@@ -375,7 +378,7 @@ struct Run {
     p.seq_base = w.seq;
     w.seq += static_cast<unsigned>(nb) + 1u;
     p.ipiv = w.d_ipiv;
-    p.margins = w.d_margins;
+    p.margins = track_margins ? w.d_margins : nullptr;
     p.ctl = w.d_ctl;
     p.slots = w.d_slots;
     p.rowk = w.d_rowk;
@@ -786,6 +789,7 @@ template <typename T>
 dlu_status execute(Run<T> &run, bool lookahead, int *info, int *ipiv, int *jpiv,
                    double *margins, dlu_report *rep) {
   Workspace &w = run.w;
+  run.track_margins = margins != nullptr;
   const int steps = run.steps;
   const hipStream_t stream = run.s;
   // `hc` stays alive until the stream has drained (its copies are ordered on the stream).

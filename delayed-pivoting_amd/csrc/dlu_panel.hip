@@ -460,18 +460,22 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
         as_word(ow, s_from[k - base]);
         ll_put_value(rk + 2 * NB, seq, ow);
       }
-      // The second-largest |value| of this workgroup's rows, for the reported margins only:
-      // published after the header, off the path every workgroup waits on.
-      uint64_t m2 = 0;
+      // The second-largest |value| of this workgroup's rows, for the reported margins only
+      // (when the caller asked for them): published after the header.
+      if (P.margins != nullptr) {
+        uint64_t m2 = 0;
 #pragma unroll
-      for (int r = 0; r < RPL; ++r) m2 = max(m2, (base + 64 * r + lane != lrow) ? key[r] : 0ull);
-      m2 = wave_max_key(m2);
-      if (lane == 0) ll_put_value(slot + 2, seq, static_cast<T>(key_value(m2)));
+        for (int r = 0; r < RPL; ++r) m2 = max(m2, (base + 64 * r + lane != lrow) ? key[r] : 0ull);
+        m2 = wave_max_key(m2);
+        if (lane == 0) ll_put_value(slot + 2, seq, static_cast<T>(key_value(m2)));
+      }
     }
     DLU_PHASE(1)
 
-    // Margin of the previous step: the largest candidate outside the winner's workgroup.
-    if (g == 0 && wave == 1 && prev_gwin >= 0) {
+    // Margin of the previous step: the largest candidate outside the winner's workgroup (only
+    // when margins were asked for: workgroup 0 is the one most often last to publish its header,
+    // and this work delays its gather).
+    if (g == 0 && wave == 1 && prev_gwin >= 0 && P.margins != nullptr) {
       constexpr int GQ = kMaxPanelGroups / 64;
       uint64_t o = 0;
 #pragma unroll
@@ -543,7 +547,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
       if (lane >= 16 && lane < 19 && c + lane - 16 < width) src = wrow + WPV * (c + lane - 16);
       if (p_here && lane >= 19 && lane < 22 && c + lane - 19 < width)
         src = rk + WPV * (c + lane - 19);
-      if (lane == 22) src = wslot + 2;
+      if (lane == 22 && P.margins != nullptr) src = wslot + 2;
       if (lane == 23) src = wslot + 4;
       if (p_here && lane == 24) src = rk + 2 * NB;
       if (p_here && lane >= 32 && lane < 48 && col0 + lane - 32 < width)
@@ -617,7 +621,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
   {
     const int steps = stop - P.pb;
     // Margin of the last eliminated step (its successor never ran).
-    if (g == 0 && wave == 1 && prev_gwin >= 0) {
+    if (g == 0 && wave == 1 && prev_gwin >= 0 && P.margins != nullptr) {
       constexpr int GQ = kMaxPanelGroups / 64;
       const int cl = steps - 1;
       uint64_t o = 0;
@@ -689,10 +693,10 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
           P.margins[P.pb + s] =
               (P.positioned && s == 0) ? -1.0 : (s_g1[s] - fmax(s_g2[s], 0.0)) / s_g1[s];
       }
-      if (wave == 0) {  // tie statistics of the column steps
+      if (wave == 0) {  // tie statistics of the column steps (when margins are tracked)
         double lo = 2.0;
         int near = 0, exact = 0;
-        for (int s0 = 0; s0 < steps; s0 += 64) {
+        for (int s0 = 0; P.margins != nullptr && s0 < steps; s0 += 64) {
           const int s = s0 + lane;
           const bool valid = s < steps && !(P.positioned && s == 0);
           const double mg = valid ? (s_g1[s] - fmax(s_g2[s], 0.0)) / s_g1[s] : 2.0;

@@ -125,6 +125,8 @@ typedef struct dlu_options {
 typedef struct dlu_report {
   int panels;       /* panel kernels launched (restarts included) */
   int fallbacks;    /* full-pivot fallbacks taken */
+  /* The tie statistics below are kept when the caller passes a margins array to
+   * dlu_getrf_device_ex (otherwise 0, 0 and 2.0): the bookkeeping costs time on every pivot step. */
   int near_ties;    /* column steps whose relative top-2 magnitude gap was below 1e-10 */
   int exact_ties;   /* column steps whose two largest magnitudes were exactly equal */
   double min_margin;/* smallest relative top-2 gap over the column steps (2.0 if none) */
