@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(BUILD, "libdlu.so")
-SOURCES = ["dlu_panel.hip", "dlu_blas3.hip", "dlu_pivot.hip", "dlu_solve.hip", "dlu_batch.hip", "dlu_front.hip", "dlu_sytrf.hip", "dlu_getrf.cpp",
+SOURCES = ["dlu_panel.hip", "dlu_panel_lag.hip", "dlu_blas3.hip", "dlu_pivot.hip", "dlu_solve.hip", "dlu_batch.hip", "dlu_front.hip", "dlu_sytrf.hip", "dlu_getrf.cpp",
            "dlu_solve.cpp", "dlu_serve.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

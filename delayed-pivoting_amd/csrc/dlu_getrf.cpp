@@ -163,6 +163,17 @@ inline void panel_shape_for(int rows, int nb, int forced_rpl, int max_groups, in
   groups = (rows + 64 * rpl - 1) / (64 * rpl);
 }
 
+// Shape of the one-exchange-per-step panel kernel: FP64 panels hold 2 rows per lane (a
+// workgroup of nine waves keeps 16 x 2 doubles per lane without spilling), FP32 panels 4.  False
+// where that kernel does not take the panel (nb 256, or more than kLagMaxGroups workgroups).
+inline bool lag_shape_for(int rows, int nb, size_t elem, int forced_rpl, int max_groups,
+                          int &groups, int &rpl) {
+  if (nb != 64 && nb != 128) return false;
+  rpl = forced_rpl > 0 ? forced_rpl : (elem == 8 ? 2 : 4);
+  groups = (rows + 64 * rpl - 1) / (64 * rpl);
+  return groups <= std::min(kLagMaxGroups, max_groups);
+}
+
 hipError_t prepare(Workspace &w, int device, int steps) {
   if (!w.ready) {
     w.device = device;
@@ -283,6 +294,7 @@ struct Run {
   int forced_split = 0;              // > 0: that many far columns always go to the panel stream
   int panel_cus = 0;                 // CUs of the panel stream (0: no partition)
   int variant = DLU_FALLBACK_FULL;   // fallback search: full pivot or rightmost nonzero column
+  int panel_kernel = DLU_PANEL_AUTO; // which register-resident panel kernel factors a panel
   hipStream_t ps = nullptr, us = nullptr;  // internal panel / update streams
 
   // Opt-in launch timing: an event pair around each launch, on the launch's own stream.
@@ -358,7 +370,10 @@ struct Run {
 
   hipError_t panel(int pb, int pe, int positioned, int parity, hipStream_t st) {
     int groups = 0, rpl = 0;
-    if (!panel_shape(pb, groups, rpl)) return tall_panel(pb, pe, positioned, parity, st);
+    const bool lag = panel_kernel != DLU_PANEL_MOVE &&
+                     lag_shape_for(m - pb, nb, sizeof(T), forced_rpl, max_groups, groups, rpl);
+    if (!lag && panel_kernel == DLU_PANEL_LAG) return hipErrorInvalidValue;
+    if (!lag && !panel_shape(pb, groups, rpl)) return tall_panel(pb, pe, positioned, parity, st);
     PanelParams<T> p{};
     p.a = a;
     p.lda = lda;
@@ -389,7 +404,7 @@ struct Run {
     p.done_seq = ++w.done_seq;
     expect_done = p.done_seq;
     hipError_t e = profile ? mark(t_panel, st, 0.0) : hipSuccess;
-    if (e == hipSuccess) e = launch_panel(p, nb, rpl, st);
+    if (e == hipSuccess) e = lag ? launch_panel_lag(p, nb, rpl, st) : launch_panel(p, nb, rpl, st);
     if (e == hipSuccess && profile) e = close(t_panel, st);
     if (e != hipSuccess) return e;
     ++panels;
@@ -870,6 +885,10 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   const int forced_rpl = opt ? opt->rows_per_lane : 0;
   if (forced_rpl != 0 && forced_rpl != 1 && forced_rpl != 2 && forced_rpl != 4)
     return DLU_INVALID_ARGUMENT;
+  const int panel_kernel = opt ? opt->panel_kernel : DLU_PANEL_AUTO;
+  if (panel_kernel != DLU_PANEL_AUTO && panel_kernel != DLU_PANEL_MOVE &&
+      panel_kernel != DLU_PANEL_LAG)
+    return DLU_INVALID_ARGUMENT;
   const bool lookahead = !(opt && opt->lookahead < 0);
   const long long spin =
       (opt && opt->spin_limit_ticks > 0) ? opt->spin_limit_ticks : 200000000ll;
@@ -912,7 +931,10 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   // could not hold them all at once (measured: 40 panel workgroups on a 48-CU mask never became
   // co-resident; on 32 and 64 CUs they do).
   int groups = 0, rpl = 0;
-  panel_shape_for(m, nb, forced_rpl, std::min(kMaxPanelGroups, cus), groups, rpl);
+  const bool lag0 = panel_kernel != DLU_PANEL_MOVE &&
+                    lag_shape_for(m, nb, sizeof(T), forced_rpl, cus, groups, rpl);
+  if (panel_kernel == DLU_PANEL_LAG && !lag0) return DLU_INVALID_ARGUMENT;
+  if (!lag0) panel_shape_for(m, nb, forced_rpl, std::min(kMaxPanelGroups, cus), groups, rpl);
   const int xcd = cus % kCusPerXcd == 0 ? kCusPerXcd : cus;  // unknown layout: no partition
   int want = 0;
   if (sizeof(T) == 4 && groups > 16)
@@ -931,6 +953,7 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   run.profile = opt && opt->profile;
   run.forced_rpl = forced_rpl;
   run.variant = variant;
+  run.panel_kernel = panel_kernel;
   if (opt && opt->merge_rows != 0) run.merge_rows = opt->merge_rows < 0 ? (1 << 30) : opt->merge_rows;
   run.split = !(opt && opt->split_update < 0);
   if (opt && opt->split_update > 0) run.forced_split = opt->split_update;

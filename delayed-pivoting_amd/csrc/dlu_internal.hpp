@@ -61,6 +61,13 @@ constexpr int kSlotHeaderWords = 8;
 constexpr unsigned kMaxStepTag = 0xFFFFu;
 inline constexpr int slot_words(int nb) { return kSlotHeaderWords + 2 * nb; }
 constexpr int kMaxPanelGroups = 256;
+// The one-exchange-per-step panel kernel (dlu_panel_lag.hip) lays the same slot memory out as
+// two step parities of kLagMaxGroups 8-word headers, then four step slots of kLagMaxGroups
+// candidate rows (two words per element, nb <= 128); it takes at most kLagMaxGroups workgroups.
+constexpr int kLagMaxGroups = 128;
+static_assert(2 * kLagMaxGroups * 8 + 4 * kLagMaxGroups * 2 * 128 <=
+                  2 * kMaxPanelGroups * (kSlotHeaderWords + 2 * 256),
+              "the lagged panel's exchange fits the slot memory");
 
 // 16 x 16 x 4 MFMA tiles in either precision (gfx950: v_mfma_f64_16x16x4_f64 and
 // v_mfma_f32_16x16x4_f32).  Operand layout, lane l of 64: A (16 x 4) holds row l % 16, k l / 16;
@@ -120,6 +127,11 @@ struct PanelParams {
 // instantiated in the translation unit that defines each launcher.
 template <typename T>
 hipError_t launch_panel(const PanelParams<T> &p, int nb, int rows_per_lane, hipStream_t s);
+// The same panel with one exchange per step (rows tracked by logical position, the bulk update
+// one step behind a leading wave); at most kLagMaxGroups workgroups, rows_per_lane <= 2 at
+// nb = 256.  Leaves the matrix, ipiv, margins, swap list and control block as launch_panel does.
+template <typename T>
+hipError_t launch_panel_lag(const PanelParams<T> &p, int nb, int rows_per_lane, hipStream_t s);
 
 // Right-side interchange + triangular solve: rows [pb, pb+w) of columns [c0, c1) become
 // U12 = L11^{-1} * (rows after interchange); rows displaced below pb+w are written directly.

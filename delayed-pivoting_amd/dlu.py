@@ -33,7 +33,12 @@ class Options(ctypes.Structure):
                 ("spin_limit_ticks", ctypes.c_longlong), ("profile", ctypes.c_int),
                 ("panel_cus", ctypes.c_int), ("rows_per_lane", ctypes.c_int),
                 ("merge_rows", ctypes.c_int), ("split_update", ctypes.c_int),
-                ("fallback", ctypes.c_int)]
+                ("fallback", ctypes.c_int), ("panel_kernel", ctypes.c_int)]
+
+
+PANEL_AUTO = 0  # one exchange per pivot step where the panel fits it
+PANEL_MOVE = 1  # the exchange-and-move panel kernel (two exchanges per step)
+PANEL_LAG = 2   # one exchange per step, or an error
 
 
 FALLBACK_FULL = 0    # delayed_getrf: largest |a| of the active block

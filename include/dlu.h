@@ -116,7 +116,16 @@ typedef struct dlu_options {
   int fallback;      /* what a zero column falls back to: DLU_FALLBACK_FULL (delayed_getrf: the
                         largest |a| of the active block) or DLU_FALLBACK_COLUMN
                         (delayed_getrf_column: the rightmost nonzero column) */
+  int panel_kernel;  /* DLU_PANEL_AUTO (0): one exchange per pivot step wherever the panel fits
+                        it (up to 128 workgroups, nb 64 or 128), else the exchange-and-move
+                        kernel; DLU_PANEL_MOVE: always the kernel that moves rows between
+                        workgroups at every step (two exchanges per step); DLU_PANEL_LAG: one
+                        exchange per step, an error where it does not fit */
 } dlu_options;
+
+#define DLU_PANEL_AUTO 0
+#define DLU_PANEL_MOVE 1
+#define DLU_PANEL_LAG 2
 
 #define DLU_FALLBACK_FULL 0
 #define DLU_FALLBACK_COLUMN 1
