@@ -69,6 +69,18 @@ struct Workspace {
   unsigned long long *d_po = nullptr;
   unsigned long long *d_phase = nullptr;
   unsigned seq = 0;
+  // Deferred left interchanges: every panel's swap list kept (archive), the composed
+  // permutation and its inverse, its copies per closed block, and the per-column plan (pinned
+  // host staging and its device copy).
+  int *d_archive = nullptr;
+  int archive_cap = 0;  // lists
+  int *d_perm = nullptr;
+  int perm_cap = 0;     // rows
+  int *d_snaps = nullptr;
+  size_t snaps_cap = 0;  // ints
+  int *d_plan = nullptr;
+  int *h_plan = nullptr;
+  int plan_cap = 0;      // ints
   // Internal streams: panels run on `panel`, everything else on `update`.  When the device
   // allows it the two streams own disjoint sets of CUs, so the panel kernel (which needs all of
   // its workgroups resident at once) never waits for trailing-update workgroups to drain.
@@ -172,6 +184,46 @@ inline bool lag_shape_for(int rows, int nb, size_t elem, int forced_rpl, int max
   rpl = forced_rpl > 0 ? forced_rpl : (elem == 8 ? 2 : 4);
   groups = (rows + 64 * rpl - 1) / (64 * rpl);
   return groups <= std::min(kLagMaxGroups, max_groups);
+}
+
+// Buffers of the deferred left interchanges for m rows, n columns and at most `lists` panels.
+hipError_t prepare_defer(Workspace &w, int m, int n, int lists) {
+  hipError_t e;
+  if (lists > w.archive_cap) {
+    hipFree(w.d_archive);
+    w.d_archive = nullptr;
+    w.archive_cap = 0;
+    if ((e = hipMalloc(&w.d_archive, static_cast<size_t>(lists) * kListStride * sizeof(int))) !=
+        hipSuccess)
+      return e;
+    w.archive_cap = lists;
+  }
+  if (m > w.perm_cap) {
+    hipFree(w.d_perm);
+    w.d_perm = nullptr;
+    w.perm_cap = 0;
+    if ((e = hipMalloc(&w.d_perm, 2ull * m * sizeof(int))) != hipSuccess) return e;
+    w.perm_cap = m;
+  }
+  const size_t snaps = static_cast<size_t>(lists) * m;
+  if (snaps > w.snaps_cap) {
+    hipFree(w.d_snaps);
+    w.d_snaps = nullptr;
+    w.snaps_cap = 0;
+    if ((e = hipMalloc(&w.d_snaps, snaps * sizeof(int))) != hipSuccess) return e;
+    w.snaps_cap = snaps;
+  }
+  const int plan = lists + 1 + 2 * n;
+  if (plan > w.plan_cap) {
+    hipFree(w.d_plan);
+    hipHostFree(w.h_plan);
+    w.d_plan = w.h_plan = nullptr;
+    w.plan_cap = 0;
+    if ((e = hipMalloc(&w.d_plan, plan * sizeof(int))) != hipSuccess) return e;
+    if ((e = hipHostMalloc(&w.h_plan, plan * sizeof(int))) != hipSuccess) return e;
+    w.plan_cap = plan;
+  }
+  return hipSuccess;
 }
 
 hipError_t prepare(Workspace &w, int device, int steps) {
@@ -295,6 +347,19 @@ struct Run {
   int panel_cus = 0;                 // CUs of the panel stream (0: no partition)
   int variant = DLU_FALLBACK_FULL;   // fallback search: full pivot or rightmost nonzero column
   int panel_kernel = DLU_PANEL_AUTO; // which register-resident panel kernel factors a panel
+  // Deferred left interchanges: while on, each panel writes its swap list into its own archive
+  // slot (panel e: event e), and the rows of the columns left of a panel are not interchanged
+  // when it ends; each closed column block records the first event it has not yet taken, and
+  // one pass at the end (or before a full-pivot fallback, which interchanges whole rows) applies
+  // the rest.  Off for single-stream runs and tall or very tall matrices.
+  bool defer = false;
+  int events = 0;
+  struct Block {
+    int c0, c1, first;  // columns [c0, c1) have taken the row movements of events < first
+  };
+  std::vector<Block> blocks;
+  const int *cur_list = nullptr;  // swap list of the panel launched last, and its event
+  int cur_event = -1;
   hipStream_t ps = nullptr, us = nullptr;  // internal panel / update streams
 
   // Opt-in launch timing: an event pair around each launch, on the launch's own stream.
@@ -351,6 +416,8 @@ struct Run {
   // all columns at once, so its swap list is the identity.  Runs in order on `st`.
   hipError_t tall_panel(int pb, int pe, int positioned, int parity, hipStream_t st) {
     int *list = w.d_lists + parity * kListStride;
+    cur_list = list;
+    cur_event = -1;
     hipError_t e = profile ? mark(t_panel, st, 0.0) : hipSuccess;
     if (e == hipSuccess) e = launch_tall_begin(w.d_ctl, pb, pe, st);
     for (int k = pb; e == hipSuccess && k < pe; ++k) {
@@ -397,7 +464,11 @@ struct Run {
     p.ctl = w.d_ctl;
     p.slots = w.d_slots;
     p.rowk = w.d_rowk;
-    p.swap_list = w.d_lists + parity * kListStride;
+    if (defer && events >= w.archive_cap) return hipErrorInvalidValue;  // (one list per panel)
+    p.swap_list = defer ? w.d_archive + static_cast<size_t>(events) * kListStride
+                        : w.d_lists + parity * kListStride;
+    cur_list = p.swap_list;
+    cur_event = defer ? events++ : -1;
     p.spin_limit = spin;
     p.phase_ticks = profile ? w.d_phase : nullptr;
     p.done = w.d_done;
@@ -490,6 +561,7 @@ struct Run {
     bool open = false;  // P0 done, its far update owed
     int b0 = 0, fe = 0;  // P0's first row/column; first column still owed P0's update
     const int *l0 = nullptr;
+    int e0 = -1;        // P0's event (deferred left interchanges)
   };
   Pair pair;
   struct Rest {  // the right half of a closed pair's far update, not yet queued
@@ -531,6 +603,7 @@ struct Run {
       }
       if ((e = gemm(r0, c0, b0, b0, m - r0, cb - c0, 2 * nb, us)) != hipSuccess) return e;
     }
+    if (defer) return hipSuccess;  // both movements reach the columns left of b0 at the end
     if ((e = launch_row_interchange(a, lda, 0, b0, rest.l0, nb, us)) != hipSuccess) return e;
     return launch_row_interchange(a, lda, 0, b0, rest.l1, nb, us);
   }
@@ -540,7 +613,51 @@ struct Run {
     pair.open = false;
     hipError_t e = wait_tail(st);
     if (e == hipSuccess) e = update(pair.b0, nb, pair.fe, n, pair.l0, st);
-    if (e == hipSuccess) e = launch_row_interchange(a, lda, 0, pair.b0, pair.l0, nb, st);
+    if (e == hipSuccess) e = left(pair.b0, pair.b0 + nb, pair.e0, pair.l0, st);
+    return e;
+  }
+
+  // The row movement of panel `ev` (its swap list) for the columns left of its block
+  // [pb, pe): interchanged now, or, deferred, recorded as the block's first owed movement.
+  hipError_t left(int pb, int pe, int ev, const int *list, hipStream_t st) {
+    if (defer) {
+      blocks.push_back(Block{pb, pe, ev + 1});
+      return hipSuccess;
+    }
+    return launch_row_interchange(a, lda, 0, pb, list, nb, st);
+  }
+
+  // The deferred left interchanges: every movement kept so far, composed backward, applied once
+  // to the rows below each recorded block.  Afterwards interchanges are immediate again.
+  hipError_t apply_deferred(hipStream_t st) {
+    if (!defer) return hipSuccess;
+    defer = false;
+    const int E = events;
+    int *snap_slot = w.h_plan;          // [0, E]
+    int *col_slot = w.h_plan + E + 1;   // [0, n)
+    int *col_r0 = col_slot + n;         // [0, n)
+    int slots = 0, ncols = 0;
+    for (int e = 0; e <= E; ++e) snap_slot[e] = -1;
+    for (const Block &b : blocks) {
+      if (b.first < E && snap_slot[b.first] < 0) snap_slot[b.first] = slots++;
+      ncols = std::max(ncols, b.c1);
+    }
+    for (int c = 0; c < ncols; ++c) col_slot[c] = -1;
+    for (const Block &b : blocks)
+      for (int c = b.c0; c < b.c1; ++c) {
+        col_slot[c] = b.first < E ? snap_slot[b.first] : -1;
+        col_r0[c] = b.c1;
+      }
+    blocks.clear();
+    if (slots == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(w.d_plan, w.h_plan, (E + 1 + 2 * static_cast<size_t>(n)) * sizeof(int),
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = launch_left_compose(m, w.d_archive, kListStride, nb, E, w.d_plan, w.d_perm,
+                              w.d_perm + m, w.d_snaps, st);
+    if (e == hipSuccess)
+      e = launch_left_gather<T>(a, lda, m, ncols, w.d_plan + E + 1, w.d_plan + E + 1 + n,
+                                w.d_snaps, st);
     return e;
   }
 
@@ -624,7 +741,8 @@ struct Run {
       if (c.error != 0) return hipErrorLaunchFailure;
       const int stop = c.stop;
       const int wdt = stop - pb;
-      const int *list = w.d_lists + parity * kListStride;
+      const int *list = cur_list;  // the panel just waited for
+      const int ev = cur_event;
       const int pe2 = std::min(pe + nb, steps);
       const bool full = !c.deferred && !c.zero_column && wdt == nb && pe - pb == nb;
       if (!single && (e = hipStreamWaitEvent(us, w.ev_ctl, 0)) != hipSuccess) return e;
@@ -654,6 +772,10 @@ struct Run {
           const int cm = std::min(n, pe2 + half);
           if ((e = pair_update(b0, pair.l0, list, pe2, cm, us)) != hipSuccess) return e;
           rest = Rest{true, b0, cm, pair.l0, list};
+          if (defer) {  // P0's columns took P1's movement above
+            blocks.push_back(Block{b0, pb, ev + 1});
+            blocks.push_back(Block{pb, pe, ev + 1});
+          }
         } else if (pe < steps && ahead && full && pe2 < n && m - pe2 >= merge_rows) {
           // First panel of a pair: only the next panel's columns now (and the previous pair's
           // right half, which the next panel's block does not touch).
@@ -664,6 +786,7 @@ struct Run {
           pair.b0 = pb;
           pair.fe = pe2;
           pair.l0 = list;
+          pair.e0 = ev;
         } else if (pe < steps && ahead) {
           // The next panel's columns first, then the far columns while the next panel runs
           // (their last columns on the panel stream behind it when the GEMM outlasts it).
@@ -679,12 +802,12 @@ struct Run {
           }
           if ((e = gemm(pb + wdt, pe2, pb, pb, m - pb - wdt, cb - pe2, wdt, us)) != hipSuccess)
             return e;
-          if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
+          if ((e = left(pb, pb + wdt, ev, list, us)) != hipSuccess) return e;
         } else {
           if ((e = queue_rest(0)) != hipSuccess) return e;
           if ((e = wait_tail(us)) != hipSuccess) return e;
           if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
-          if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
+          if ((e = left(pb, pb + wdt, ev, list, us)) != hipSuccess) return e;
           if (pe >= steps) {
             *info = steps;
             break;
@@ -700,8 +823,10 @@ struct Run {
         if ((e = wait_tail(us)) != hipSuccess) return e;
         if (wdt > 0) {
           if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
-          if ((e = launch_row_interchange(a, lda, 0, pb, list, nb, us)) != hipSuccess) return e;
+          if ((e = left(pb, pb + wdt, ev, list, us)) != hipSuccess) return e;
         }
+        // The fallback interchanges whole rows: every owed left movement goes first.
+        if ((e = apply_deferred(us)) != hipSuccess) return e;
         if (profile &&
             (e = mark(t_search, us, 0.0,
                       sizeof(T) * static_cast<double>(m - stop) * static_cast<double>(n - stop))) !=
@@ -730,7 +855,7 @@ struct Run {
       parity = (parity + 1) % kLists;
       if ((e = launch_tail()) != hipSuccess) return e;
     }
-    return hipSuccess;
+    return apply_deferred(us);
   }
 
   // The caller's stream resumes after everything queued on the internal streams.
@@ -757,8 +882,11 @@ void release_workspace(Workspace &w) {
                   static_cast<void *>(w.d_jpiv), static_cast<void *>(w.d_margins),
                   static_cast<void *>(w.d_slots), static_cast<void *>(w.d_rowk),
                   static_cast<void *>(w.d_lists), static_cast<void *>(w.d_pv),
-                  static_cast<void *>(w.d_po), static_cast<void *>(w.d_phase)})
+                  static_cast<void *>(w.d_po), static_cast<void *>(w.d_phase),
+                  static_cast<void *>(w.d_archive), static_cast<void *>(w.d_perm),
+                  static_cast<void *>(w.d_snaps), static_cast<void *>(w.d_plan)})
     if (p) hipFree(p);
+  if (w.h_plan) hipHostFree(w.h_plan);
   if (w.h_ctl) hipHostFree(w.h_ctl);
   if (w.h_done) hipHostFree(w.h_done);
   w = Workspace{};
@@ -817,6 +945,9 @@ dlu_status execute(Run<T> &run, bool lookahead, int *info, int *ipiv, int *jpiv,
   if ((e = hipMemcpyAsync(w.d_ctl, &hc, sizeof(Control), hipMemcpyHostToDevice, stream)) ==
           hipSuccess &&
       (e = hipMemsetAsync(w.d_lists, 0, kLists * kListStride * sizeof(int), stream)) == hipSuccess &&
+      (!run.defer || (e = hipMemsetAsync(w.d_archive, 0,
+                                         static_cast<size_t>(w.archive_cap) * kListStride * sizeof(int),
+                                         stream)) == hipSuccess) &&
       (e = launch_init_steps(w.d_ipiv, w.d_jpiv, w.d_margins, steps, stream)) == hipSuccess) {
     int out_info = steps;
     e = run.factor(&out_info, lookahead);
@@ -973,6 +1104,12 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
                          hipSuccess) {
     *info = -1;
     return to_status(e);
+  }
+  // Left interchanges deferred to one pass at the end: the panels' lists stay (one per panel),
+  // and each column is permuted once (through LDS: at most kDeferMaxRows rows).
+  if (m <= kDeferMaxRows && !run.tall(0) && !(opt && opt->defer_left < 0)) {
+    const int lists = (steps + nb - 1) / nb;
+    run.defer = prepare_defer(w, m, n, lists) == hipSuccess && w.archive_cap >= lists;
   }
   return execute(run, lookahead, info, ipiv, jpiv, margins, rep);
 }

@@ -159,6 +159,17 @@ template <typename T>
 hipError_t launch_row_interchange(T *a, int lda, int c0, int c1, const int *swap_list, int nb,
                                   hipStream_t s);
 
+// Deferred left interchanges (factorizations of at most kDeferMaxRows rows): compose the kept
+// net row movements of `events` panels backward, writing the permutation into snaps[slot] at
+// every e with snap_slot[e] = slot >= 0 (one workgroup; P and Pinv hold m ints each), then
+// gather rows [col_r0[c], m) of each column c < ncols with col_slot[c] >= 0 by snaps[slot].
+constexpr int kDeferMaxRows = 16384;
+hipError_t launch_left_compose(int m, const int *lists, int stride, int nb, int events,
+                               const int *snap_slot, int *P, int *Pinv, int *snaps, hipStream_t s);
+template <typename T>
+hipError_t launch_left_gather(T *a, int lda, int m, int ncols, const int *col_slot,
+                              const int *col_r0, const int *snaps, hipStream_t s);
+
 // Identity swap histories and "unreached" margins for a new factorization.
 hipError_t launch_init_steps(int *ipiv, int *jpiv, double *margins, int steps, hipStream_t s);
 

@@ -364,6 +364,64 @@ __global__ __launch_bounds__(256) void row_interchange_kernel(T *__restrict__ a,
   }
 }
 
+// Deferred left interchanges.  The panels' net row movements are kept in order (list e: row
+// dst[k] receives the content of row src[k]); after the last panel one workgroup composes,
+// from the last movement backward, the permutation P (row i of a block closed before movement
+// e ends up with the content row P(i) holds now: P <- q_e o P, with q_e(dst) = src) and copies
+// P out wherever a block needs it.  Each list moves a set of rows onto itself, so the update of
+// P is two parallel phases through its inverse.  Rows below a block's last row are the only
+// ones the later movements touch.
+__global__ __launch_bounds__(1024) void left_compose_kernel(int m, const int *__restrict__ lists,
+                                                           int stride, int nb, int events,
+                                                           const int *__restrict__ snap_slot,
+                                                           int *P, int *Pinv, int *snaps) {
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    P[i] = i;
+    Pinv[i] = i;
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (int e = events; e >= 1; --e) {
+    const int slot = snap_slot[e];
+    if (slot >= 0)
+      for (int i = threadIdx.x; i < m; i += blockDim.x) snaps[static_cast<size_t>(slot) * m + i] = P[i];
+    const int *L = lists + static_cast<size_t>(e - 1) * stride;
+    const int cnt = L[4 * nb];  // at most 2 nb <= 512 entries: one per thread
+    const bool mine = static_cast<int>(threadIdx.x) < cnt;
+    int at = 0, src = 0;
+    if (mine) {
+      at = Pinv[L[threadIdx.x]];
+      src = L[2 * nb + threadIdx.x];
+    }
+    __syncthreads();
+    if (mine) {
+      P[at] = src;
+      Pinv[src] = at;
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// One workgroup per column of the deferred blocks: rows [r0, m) of the column, staged in LDS,
+// are written back in the order of the block's composed permutation.
+template <typename T>
+__global__ __launch_bounds__(1024) void left_gather_kernel(T *a, int lda, int m,
+                                                          const int *__restrict__ col_slot,
+                                                          const int *__restrict__ col_r0,
+                                                          const int *__restrict__ snaps) {
+  __shared__ T seg[kDeferMaxRows];
+  const int c = blockIdx.x;
+  const int slot = col_slot[c];
+  if (slot < 0) return;
+  const int r0 = col_r0[c];
+  T *cp = a + static_cast<size_t>(c) * lda;
+  for (int i = r0 + threadIdx.x; i < m; i += blockDim.x) seg[i - r0] = cp[i];
+  __syncthreads();
+  const int *Pc = snaps + static_cast<size_t>(slot) * m;
+  for (int i = r0 + threadIdx.x; i < m; i += blockDim.x) cp[i] = seg[Pc[i] - r0];
+}
+
 __global__ void init_steps_kernel(int *ipiv, int *jpiv, double *margins, int steps) {
   for (int k = blockIdx.x * 256 + threadIdx.x; k < steps; k += gridDim.x * 256) {
     ipiv[k] = k;
@@ -443,6 +501,24 @@ hipError_t launch_fallback_apply(T *a, int lda, int m, int n, int k, int *ipiv, 
   return hipGetLastError();
 }
 
+hipError_t launch_left_compose(int m, const int *lists, int stride, int nb, int events,
+                               const int *snap_slot, int *P, int *Pinv, int *snaps,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(left_compose_kernel, dim3(1), dim3(1024), 0, s, m, lists, stride, nb, events,
+                     snap_slot, P, Pinv, snaps);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_left_gather(T *a, int lda, int m, int ncols, const int *col_slot,
+                              const int *col_r0, const int *snaps, hipStream_t s) {
+  if (ncols <= 0) return hipSuccess;
+  if (m > kDeferMaxRows) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(left_gather_kernel<T>, dim3(ncols), dim3(1024), 0, s, a, lda, m, col_slot,
+                     col_r0, snaps);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_row_interchange(T *a, int lda, int c0, int c1, const int *swap_list, int nb,
                                   hipStream_t s) {
@@ -463,6 +539,8 @@ hipError_t launch_row_interchange(T *a, int lda, int c0, int c1, const int *swap
   template hipError_t launch_fallback_apply<T>(T *, int, int, int, int, int *, int *, Control *, \
                                                hipStream_t);                                     \
   template hipError_t launch_row_interchange<T>(T *, int, int, int, const int *, int, hipStream_t); \
+  template hipError_t launch_left_gather<T>(T *, int, int, int, const int *, const int *,         \
+                                            const int *, hipStream_t);                            \
   template hipError_t launch_column_search<T>(const T *, int, int, int, int, Control *,          \
                                               hipStream_t);                                     \
   template hipError_t launch_tall_pivot<T>(T *, int, int, int, int *, Control *, hipStream_t);   \

@@ -33,7 +33,8 @@ class Options(ctypes.Structure):
                 ("spin_limit_ticks", ctypes.c_longlong), ("profile", ctypes.c_int),
                 ("panel_cus", ctypes.c_int), ("rows_per_lane", ctypes.c_int),
                 ("merge_rows", ctypes.c_int), ("split_update", ctypes.c_int),
-                ("fallback", ctypes.c_int), ("panel_kernel", ctypes.c_int)]
+                ("fallback", ctypes.c_int), ("panel_kernel", ctypes.c_int),
+                ("defer_left", ctypes.c_int)]
 
 
 PANEL_AUTO = 0  # one exchange per pivot step where the panel fits it

@@ -121,6 +121,9 @@ typedef struct dlu_options {
                         kernel; DLU_PANEL_MOVE: always the kernel that moves rows between
                         workgroups at every step (two exchanges per step); DLU_PANEL_LAG: one
                         exchange per step, an error where it does not fit */
+  int defer_left;    /* row interchanges of the columns left of each panel: 0 (default) = once
+                        at the end, composed (up to 16384 rows, until a full-pivot fallback),
+                        < 0 = as each panel ends */
 } dlu_options;
 
 #define DLU_PANEL_AUTO 0
