@@ -167,7 +167,7 @@ class Lease {
 inline void panel_shape_for(int rows, int nb, int forced_rpl, int max_groups, int &groups,
                             int &rpl) {
   if (forced_rpl > 0) {
-    rpl = forced_rpl;
+    rpl = forced_rpl == 3 ? 4 : forced_rpl;  // 3 rows per lane: the one-exchange kernel only
   } else {
     rpl = nb == 256 ? 2 : 4;
     if (nb == 256 && (rows + 63) / 64 <= max_groups) rpl = 1;  // avoid the spilling variant
@@ -1014,8 +1014,7 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   const int variant = opt ? opt->fallback : DLU_FALLBACK_FULL;
   if (variant != DLU_FALLBACK_FULL && variant != DLU_FALLBACK_COLUMN) return DLU_INVALID_ARGUMENT;
   const int forced_rpl = opt ? opt->rows_per_lane : 0;
-  if (forced_rpl != 0 && forced_rpl != 1 && forced_rpl != 2 && forced_rpl != 4)
-    return DLU_INVALID_ARGUMENT;
+  if (forced_rpl < 0 || forced_rpl > 4) return DLU_INVALID_ARGUMENT;
   const int panel_kernel = opt ? opt->panel_kernel : DLU_PANEL_AUTO;
   if (panel_kernel != DLU_PANEL_AUTO && panel_kernel != DLU_PANEL_MOVE &&
       panel_kernel != DLU_PANEL_LAG)

@@ -123,8 +123,19 @@ __device__ __forceinline__ T &col_at(T *colp, int row) {
                                 static_cast<unsigned>(row) * static_cast<unsigned>(sizeof(T)));
 }
 
+// Rows per lane an owner wave keeps in registers; the rest (FP64 beyond two) live in LDS, which
+// keeps the owners within the register share of a nine-wave workgroup.
+template <typename T, int RPL>
+struct OwnerRows {
+  static constexpr int regs = (sizeof(T) == 8 && RPL > 2) ? 2 : RPL;
+  static constexpr int lds = RPL - regs;
+};
+
 template <typename T, int NB, int RPL>
 struct LagShared {
+  static constexpr int RL = OwnerRows<T, RPL>::lds;
+  // Owner rows kept in LDS: [wave][row - regs][column][lane], one 64-lane run per column.
+  T vl[RL > 0 ? NB / 16 : 1][RL > 0 ? RL : 1][RL > 0 ? 16 : 1][RL > 0 ? 64 : 1];
   T l[2][64 * RPL];      // multipliers of step t (leader -> owners), by step parity
   T h[2][64 * RPL];      // window column handed to the leader (owners -> leader)
   T u[2][NB];            // pivot row of step t (the owners' fetch; the leader reads one value)
@@ -171,15 +182,6 @@ __device__ __forceinline__ void lds_signal(int *flag, int value) {
   __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// One value of x[RPL] picked by a wave-uniform index.
-template <typename T, int RPL>
-__device__ __forceinline__ T pick(const T (&x)[RPL], int r) {
-  T s = x[0];
-#pragma unroll
-  for (int i = 1; i < RPL; ++i) s = (r == i) ? x[i] : s;
-  return s;
-}
-
 template <typename T, int NB, int RPL>
 struct Lag {
   // Sixteen columns of one row as a register vector (a select over its elements by a uniform
@@ -187,6 +189,8 @@ struct Lag {
   typedef T row16_t __attribute__((ext_vector_type(16)));
   static constexpr int WPV = Words<T>::n;
   static constexpr int R = 64 * RPL;
+  static constexpr int RR = OwnerRows<T, RPL>::regs;  // rows in registers (v)
+  static constexpr int RL = OwnerRows<T, RPL>::lds;   // rows in LDS (vl)
 
   PanelParams<T> P;  // a copy: a reference would keep the kernel arguments in scratch memory
   LagShared<T, NB, RPL> &S;
@@ -194,11 +198,13 @@ struct Lag {
   long long t0;
 
   __device__ unsigned seq(int t) const { return P.seq_base + static_cast<unsigned>(t) + 1u; }
+  // Row RR + rr of this lane, column j, in LDS.
+  __device__ __forceinline__ T &vl(int rr, int j) { return S.vl[wave][rr][j][lane]; }
 
   // The owners' part of step t on this wave's columns: fetch the pivot row's U part, write the
   // pivot row out when this workgroup holds it, apply the rank-1 update with the leader's
   // multipliers, and hand the leader column t+4 (through step t).
-  __device__ __forceinline__ void apply(int t, row16_t (&v)[RPL]) {
+  __device__ __forceinline__ void apply(int t, row16_t (&v)[RR]) {
     const int gwin = S.win[t & 1][0];
     const bool has_u = col0 + 15 > t;  // a column of this wave lies right of t
     uint64_t got = 0;
@@ -231,24 +237,32 @@ struct Lag {
     // with the leader's wait rather than with its search.
     if (t + 1 < width && !lds_wait(&S.flag_c, t + 1, &S.fail, P.ctl, P.spin_limit, t0)) return;
     if (!lds_wait(&S.flag_l, t, &S.fail, P.ctl, P.spin_limit, t0)) return;
-    T l[RPL];
+    T l[RR];  // the LDS rows read theirs where they use it
 #pragma unroll
-    for (int r = 0; r < RPL; ++r) l[r] = S.l[t & 1][64 * r + lane];
+    for (int r = 0; r < RR; ++r) l[r] = S.l[t & 1][64 * r + lane];
+    const T *lx = &S.l[t & 1][64 * RR + lane];
     if (jp < 0) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const T uj = u[j];
 #pragma unroll
-        for (int r = 0; r < RPL; ++r) v[r][j] = fma(-l[r], uj, v[r][j]);
+        for (int r = 0; r < RR; ++r) v[r][j] = fma(-l[r], uj, v[r][j]);
+#pragma unroll
+        for (int rr = 0; rr < RL; ++rr) vl(rr, j) = fma(-lx[64 * rr], uj, vl(rr, j));
       }
     } else {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const T uj = u[j];
 #pragma unroll
-        for (int r = 0; r < RPL; ++r) {
+        for (int r = 0; r < RR; ++r) {
           const T f = fma(-l[r], uj, v[r][j]);
           v[r][j] = (j == jp) ? l[r] : ((j > jp) ? f : v[r][j]);
+        }
+#pragma unroll
+        for (int rr = 0; rr < RL; ++rr) {
+          if (j > jp) vl(rr, j) = fma(-lx[64 * rr], uj, vl(rr, j));
+          if (j == jp) vl(rr, j) = lx[64 * rr];
         }
       }
     }
@@ -256,18 +270,20 @@ struct Lag {
     if (th < width && (th >> 4) == wave) {
       const int jh = th & 15;
 #pragma unroll
-      for (int r = 0; r < RPL; ++r) {
+      for (int r = 0; r < RR; ++r) {
         T x = v[r][0];
 #pragma unroll
         for (int j = 1; j < 16; ++j) x = (j == jh) ? v[r][j] : x;
         S.h[(t + 1) & 1][64 * r + lane] = x;
       }
+#pragma unroll
+      for (int rr = 0; rr < RL; ++rr) S.h[(t + 1) & 1][64 * (RR + rr) + lane] = vl(rr, jh);
     }
   }
 
   // Stage this wave's slice of the candidate row of step t (through step t-1) in LDS and publish
   // it when it holds columns right of t.
-  __device__ __forceinline__ void publish(int t, const row16_t (&v)[RPL]) {
+  __device__ __forceinline__ void publish(int t, const row16_t (&v)[RR]) {
     if (t >= width || col0 + 15 < t) return;
     if (!lds_wait(&S.flag_c, t, &S.fail, P.ctl, P.spin_limit, t0)) return;
     const int cl = S.cand[t & 1];
@@ -275,10 +291,16 @@ struct Lag {
     // (a per-lane test of the held row: a test of the uniform index alone lets the compiler
     // index the register array dynamically, through scratch memory)
 #pragma unroll
-    for (int r = 0; r < RPL; ++r)
+    for (int r = 0; r < RR; ++r)
       if (64 * r + lane == cl) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) S.stage[wave][j] = v[r][j];
+      }
+#pragma unroll
+    for (int rr = 0; rr < RL; ++rr)
+      if (64 * (RR + rr) + lane == cl) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) S.stage[wave][j] = vl(rr, j);
       }
     wave_sync_lds();
     if (col0 + 15 > t && lane < 16 * WPV) {
@@ -315,21 +337,28 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
   __shared__ LagShared<T, NB, RPL> S;
 
   constexpr int NW = NB / 16;  // owner waves; wave NW leads
-  row16_t v[RPL];
+  constexpr int RR = OwnerRows<T, RPL>::regs;
+  row16_t v[RR];
   if (wave < NW) {
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
       const int row = base + 64 * r + lane;
+      T x[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         T *colp = P.a + static_cast<size_t>(P.pb + col0 + j) * P.lda;  // uniform
-        v[r][j] = (row < m && col0 + j < width) ? col_at(colp, row) : T(0);
+        x[j] = (row < m && col0 + j < width) ? col_at(colp, row) : T(0);
+      }
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if (r < RR) v[r < RR ? r : 0][j] = x[j];
+        else S.vl[wave][r - RR][j][lane] = x[j];
       }
       if (wave == 0) {  // the leader's first window: columns 0-2, and column 3 handed over
-        S.l[0][64 * r + lane] = v[r][0];
-        S.l[1][64 * r + lane] = v[r][1];
-        S.h[1][64 * r + lane] = v[r][2];
-        S.h[0][64 * r + lane] = v[r][3];
+        S.l[0][64 * r + lane] = x[0];
+        S.l[1][64 * r + lane] = x[1];
+        S.h[1][64 * r + lane] = x[2];
+        S.h[0][64 * r + lane] = x[3];
       }
     }
   }
@@ -392,7 +421,8 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
 #pragma unroll
           for (int j = 0; j < 16; ++j)
             if (col0 + j < width)
-              col_at(P.a + static_cast<size_t>(P.pb + col0 + j) * P.lda, row) = v[r][j];
+              col_at(P.a + static_cast<size_t>(P.pb + col0 + j) * P.lda, row) =
+                  r < RR ? v[r < RR ? r : 0][j] : S.vl[wave][r < RR ? 0 : r - RR][j][lane];
         }
       }
     }
@@ -403,8 +433,12 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
     // rest of the window and the owners' inputs while the header travels, then every header in
     // and the winner of step t.
     __builtin_amdgcn_s_setprio(2);
-    T W0[RPL], W1[RPL], W2[RPL], Lp[RPL];  // the window of the step whose header goes out next
-    int pos[RPL];
+    // Per-lane values of the RPL held rows as register vectors: the candidate's element is read
+    // by a uniform index, which a vector keeps in registers (an array would go to scratch).
+    typedef T tvec __attribute__((ext_vector_type(RPL)));
+    typedef int ivec __attribute__((ext_vector_type(RPL)));
+    tvec W0, W1, W2, Lp;  // the window of the step whose header goes out next
+    ivec pos;
     unsigned elig = 0;
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
@@ -433,7 +467,7 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
     bool zero_now = false;
     for (int t = 0;; ++t) {
       const int k = P.pb + t;
-      T l[RPL], n0[RPL];  // multipliers of step t-1; column t through step t-1
+      tvec l, n0;  // multipliers of step t-1; column t through step t-1
       if (t > 0) {
         if (__hip_atomic_load(&S.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) failed = true;
         if (lane == 0) {
@@ -488,7 +522,7 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
               cl = __ffsll(static_cast<long long>(b)) - 1;
             }
           }
-          cpos = __builtin_amdgcn_readlane(pick(pos, cr), cl);
+          cpos = __builtin_amdgcn_readlane(pos[cr], cl);
         } else {
           int mp = INT_MAX;
 #pragma unroll
@@ -526,16 +560,16 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
       // The header: the candidate's window values, computed for it alone first.
       {
         const unsigned sq = P.seq_base + static_cast<unsigned>(t) + 1u;
-        const T c0 = lane_value(pick(n0, cr), cl);
+        const T c0 = lane_value(n0[cr], cl);
         T c1, c2, c3;
         if (t > 0) {
-          const T cw2 = lane_value(pick(W2, cr), cl), clp = lane_value(pick(Lp, cr), cl);
-          c3 = lane_value(pick(l, cr), cl);
+          const T cw2 = lane_value(W2[cr], cl), clp = lane_value(Lp[cr], cl);
+          c3 = lane_value(l[cr], cl);
           c1 = fma(-c3, u2, lag ? fma(-clp, up, cw2) : cw2);
           c2 = t + 2 < width ? S.h[(t - 1) & 1][64 * cr + cl] : T(0);
         } else {
-          c1 = lane_value(pick(W1, cr), cl);
-          c2 = lane_value(pick(W2, cr), cl);
+          c1 = lane_value(W1[cr], cl);
+          c2 = lane_value(W2[cr], cl);
           c3 = T(0);
         }
         unsigned h[NHALF];

@@ -106,7 +106,8 @@ typedef struct dlu_options {
   int panel_cus;   /* CUs reserved for panel kernels while updates run on the rest: > 0 = that
                       many, < 0 = none, 0 = automatic (none for FP64; for FP32 above 4096 rows,
                       one CU per panel workgroup, at most half the device) */
-  int rows_per_lane; /* panel rows held per lane: 0 = automatic, else 1, 2 or 4 */
+  int rows_per_lane; /* panel rows held per lane: 0 = automatic, else 1 to 4 (the exchange-and-move
+                        panel rounds 3 up to 4) */
   int merge_rows;    /* pairs of panels share one trailing GEMM (K = 2 nb) while at least this
                         many rows remain below them: 0 = automatic (3000), < 0 = never */
   int split_update;  /* the far trailing GEMM's last columns run on the panel stream once the
