@@ -618,9 +618,10 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
       lds_signal(&S.flag_c, t);  // the candidate, and the header is out
       DLU_PHASE(3)
 
-      // Every workgroup's header: lane i reads workgroups i and i + 64.  Spin on the first 16
-      // bytes of each (one load: the pivot candidate and its position), then fetch the rest
-      // while the winner is chosen.
+      // Every workgroup's header: lane i reads workgroups i and i + 64.  Each poll loads the
+      // whole header (its words arrive together, one store of the writer) and tests the first
+      // 16 bytes (the pivot candidate and its position); the winner's other words are then
+      // already in registers, not one more device-scope round trip away.
       if (!failed) {
         const bool q0 = lane < G, q1 = GQ > 1 && lane + 64 < G;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
@@ -637,21 +638,18 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
                  static_cast<unsigned>(w[j + 1] >> 48) == tg;
         };
         for (int it = 1;; ++it) {
-          if (q0) ld_pair(rs, o0, hw0[0], hw0[1]);
-          if (q1) ld_pair(rs, o1, hw1[0], hw1[1]);
+#pragma unroll
+          for (int j = 0; j < NHW; j += 2)
+            if (j < nw) {
+              if (q0) ld_pair(rs, o0 + 8 * j, hw0[j], hw0[j + 1]);
+              if (q1) ld_pair(rs, o1 + 8 * j, hw1[j], hw1[j + 1]);
+            }
           if (__all((!q0 || fresh2(hw0, 0)) && (!q1 || fresh2(hw1, 0)))) break;
           if ((it & 31) == 0 && expired(P.ctl, P.spin_limit, t0)) {
             failed = true;
             break;
           }
         }
-        // The rest of every header, in flight during the choice of the winner.
-#pragma unroll
-        for (int j = 2; j < NHW; j += 2)
-          if (j < nw) {
-            if (q0) ld_pair(rs, o0 + 8 * j, hw0[j], hw0[j + 1]);
-            if (q1) ld_pair(rs, o1 + 8 * j, hw1[j], hw1[j + 1]);
-          }
         DLU_PHASE(4)
         if (!failed) {
           unsigned a0[3], a1[3];  // pair 0: W0 and the position

@@ -94,7 +94,7 @@ def test_lag_equals_move_fp32(case):
     assert np.array_equal(ll[0], lm[0], equal_nan=True)
 
 
-@pytest.mark.parametrize("rpl", [1, 2, 4])
+@pytest.mark.parametrize("rpl", [1, 2, 3, 4])
 def test_lag_rows_per_lane(rpl):
     a = _case("zero_cols", seed=5)
     lm = _factor(a, dlu.PANEL_MOVE)
