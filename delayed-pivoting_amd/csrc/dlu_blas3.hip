@@ -38,7 +38,8 @@ __global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, i
                                                          const T *__restrict__ A, int lda,
                                                          const T *__restrict__ B, int ldb,
                                                          int M, int N, int K, int tiles_m,
-                                                         int tiles_n) {
+                                                         int tiles_n, Pred pred) {
+  if (skipped(pred)) return;
   typedef Mfma<T> MF;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
@@ -227,7 +228,9 @@ __global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, i
 template <typename T, int NB, bool LIST, bool FULL>
 __global__ __launch_bounds__(64) void swap_trsm_kernel(T *__restrict__ a, int lda, int pb, int w,
                                                       int c0, int c1,
-                                                      const int *__restrict__ swap_list) {
+                                                      const int *__restrict__ swap_list,
+                                                      Pred pred) {
+  if (skipped(pred)) return;
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
   constexpr int NT = NB / 16;  // 16-row tiles of the panel
@@ -378,7 +381,7 @@ __global__ __launch_bounds__(64) void swap_trsm_kernel(T *__restrict__ a, int ld
 
 template <typename T, int BM, int BN, int BK, int OCC>
 hipError_t launch_gemm_tile(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
-                            int K, hipStream_t s) {
+                            int K, hipStream_t s, Pred pred) {
   const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   // 16-byte loads: 16-byte aligned origins, leading dimensions, M and K multiples of the width.
   constexpr int vw = 16 / static_cast<int>(sizeof(T));
@@ -387,10 +390,10 @@ hipError_t launch_gemm_tile(T *c, int ldc, const T *a, int lda, const T *b, int 
                    (reinterpret_cast<uintptr_t>(b) % 16 == 0);
   if (vec)
     hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK, true, OCC>), dim3(tm * tn), dim3(256), 0,
-                       s, c, ldc, a, lda, b, ldb, M, N, K, tm, tn);
+                       s, c, ldc, a, lda, b, ldb, M, N, K, tm, tn, pred);
   else
     hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK, false, OCC>), dim3(tm * tn), dim3(256), 0,
-                       s, c, ldc, a, lda, b, ldb, M, N, K, tm, tn);
+                       s, c, ldc, a, lda, b, ldb, M, N, K, tm, tn, pred);
   return hipGetLastError();
 }
 
@@ -401,12 +404,12 @@ namespace {
 // The tile configurations of the trailing-update GEMM (workgroup tile, workgroups per CU).
 template <typename T>
 hipError_t launch_gemm_config(int config, T *c, int ldc, const T *a, int lda, const T *b,
-                              int ldb, int M, int N, int K, hipStream_t s) {
+                              int ldb, int M, int N, int K, hipStream_t s, Pred pred) {
   switch (config) {
-    case 0: return launch_gemm_tile<T, 64, 64, 16, 4>(c, ldc, a, lda, b, ldb, M, N, K, s);
-    case 1: return launch_gemm_tile<T, 128, 128, 16, 2>(c, ldc, a, lda, b, ldb, M, N, K, s);
-    case 2: return launch_gemm_tile<T, 128, 64, 16, 3>(c, ldc, a, lda, b, ldb, M, N, K, s);
-    case 3: return launch_gemm_tile<T, 64, 128, 16, 3>(c, ldc, a, lda, b, ldb, M, N, K, s);
+    case 0: return launch_gemm_tile<T, 64, 64, 16, 4>(c, ldc, a, lda, b, ldb, M, N, K, s, pred);
+    case 1: return launch_gemm_tile<T, 128, 128, 16, 2>(c, ldc, a, lda, b, ldb, M, N, K, s, pred);
+    case 2: return launch_gemm_tile<T, 128, 64, 16, 3>(c, ldc, a, lda, b, ldb, M, N, K, s, pred);
+    case 3: return launch_gemm_tile<T, 64, 128, 16, 3>(c, ldc, a, lda, b, ldb, M, N, K, s, pred);
     default: return hipErrorInvalidValue;
   }
 }
@@ -422,33 +425,33 @@ int tuned_gemm_config(int M, int N, int K) {
 
 template <typename T>
 hipError_t launch_gemm_sub(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
-                           int K, hipStream_t s, int config) {
+                           int K, hipStream_t s, int config, Pred pred) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
   // The tile configuration comes from the table the autotuner wrote (by depth and launch size)
   // unless the caller names one.
   return launch_gemm_config<T>(config >= 0 ? config : tuned_gemm_config(M, N, K), c, ldc, a, lda,
-                               b, ldb, M, N, K, s);
+                               b, ldb, M, N, K, s, pred);
 }
 
 template <typename T>
 hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const int *swap_list,
-                            int nb, hipStream_t s) {
+                            int nb, hipStream_t s, Pred pred) {
   if (c1 <= c0 || w <= 0) return hipSuccess;
   const dim3 grid((c1 - c0 + 15) / 16), block(64);
 #define DLU_TRSM_CASE(NBV)                                                                      \
   case NBV:                                                                                      \
     if (swap_list && w == NBV)                                                                   \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, true, true>), grid, block, 0, s, a, lda, pb,  \
-                         w, c0, c1, swap_list);                                                  \
+                         w, c0, c1, swap_list, pred);                                                  \
     else if (swap_list)                                                                          \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, true, false>), grid, block, 0, s, a, lda, pb, \
-                         w, c0, c1, swap_list);                                                  \
+                         w, c0, c1, swap_list, pred);                                                  \
     else if (w == NBV)                                                                           \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, false, true>), grid, block, 0, s, a, lda, pb, \
-                         w, c0, c1, swap_list);                                                  \
+                         w, c0, c1, swap_list, pred);                                                  \
     else                                                                                         \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, false, false>), grid, block, 0, s, a, lda,    \
-                         pb, w, c0, c1, swap_list);                                              \
+                         pb, w, c0, c1, swap_list, pred);                                              \
     break;
   switch (nb) {
     DLU_TRSM_CASE(64)
@@ -462,9 +465,9 @@ hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const 
 
 #define DLU_BLAS3_INSTANTIATE(T)                                                                 \
   template hipError_t launch_gemm_sub<T>(T *, int, const T *, int, const T *, int, int, int, int, \
-                                         hipStream_t, int);                                      \
+                                         hipStream_t, int, Pred);                                \
   template hipError_t launch_swap_trsm<T>(T *, int, int, int, int, int, const int *, int,        \
-                                          hipStream_t);
+                                          hipStream_t, Pred);
 DLU_BLAS3_INSTANTIATE(float)
 DLU_BLAS3_INSTANTIATE(double)
 #undef DLU_BLAS3_INSTANTIATE

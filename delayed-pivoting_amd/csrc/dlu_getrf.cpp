@@ -56,7 +56,9 @@ struct Workspace {
   int cus = 0;
   Control *d_ctl = nullptr;
   Control *h_ctl = nullptr;
-  PanelDone *h_done = nullptr;   // pinned, coherent: written by panel kernels
+  // Pinned, coherent: panel launch `seq` publishes into slot seq % kDoneSlots (a panel queued on
+  // speculation may finish before the host has read the one ahead of it).
+  PanelDone *h_done = nullptr;
   PanelDone *d_done = nullptr;   // the same memory as the device sees it
   unsigned long long done_seq = 0;
   int *d_ipiv = nullptr;
@@ -91,8 +93,10 @@ struct Workspace {
   Streams streams[4];
   int nstreams = 0;
   int next_replace = 0;  // slot replaced when all four hold other partitions
-  hipEvent_t ev_fork = nullptr, ev_ctl = nullptr, ev_next = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_next = nullptr, ev_join = nullptr;
   hipEvent_t ev_solved = nullptr, ev_tail = nullptr;  // far U block row done / tail GEMM done
+  hipEvent_t ev_pan[4] = {};                          // panel launch seq ends: ev_pan[seq % 4]
+  hipEvent_t ev_near = nullptr, ev_far = nullptr;     // panel-stream near work / update-stream work
   bool ready = false;
 };
 
@@ -101,6 +105,7 @@ constexpr int kCusPerXcd = 32;  // MI355X (gfx950): 8 XCDs of 32 CUs
 constexpr int kSearchPartials = 2048;
 constexpr int kListStride = 4 * 256 + 4;  // swap_list_ints(256), enough for every nb
 constexpr int kLists = 4;  // panels j and j+4 share a list (a pair's far update reads two)
+constexpr int kDoneSlots = 4;  // host-visible control block copies, by panel launch
 
 // Exchange buffers, sized for the largest panel width (256) and workgroup count.
 constexpr size_t slot_bytes() {
@@ -238,10 +243,10 @@ hipError_t prepare(Workspace &w, int device, int steps) {
     w.cus = prop.multiProcessorCount;
     if ((e = hipMalloc(&w.d_ctl, sizeof(Control))) != hipSuccess) return e;
     if ((e = hipHostMalloc(&w.h_ctl, sizeof(Control))) != hipSuccess) return e;
-    if ((e = hipHostMalloc(reinterpret_cast<void **>(&w.h_done), sizeof(PanelDone),
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&w.h_done), kDoneSlots * sizeof(PanelDone),
                            hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess)
       return e;
-    std::memset(static_cast<void *>(w.h_done), 0, sizeof(PanelDone));
+    std::memset(static_cast<void *>(w.h_done), 0, kDoneSlots * sizeof(PanelDone));
     if ((e = hipHostGetDevicePointer(reinterpret_cast<void **>(&w.d_done), w.h_done, 0)) !=
         hipSuccess)
       return e;
@@ -253,7 +258,9 @@ hipError_t prepare(Workspace &w, int device, int steps) {
     if ((e = hipMalloc(&w.d_pv, kSearchPartials * sizeof(double))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_po, kSearchPartials * sizeof(unsigned long long))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_phase, 8 * sizeof(unsigned long long))) != hipSuccess) return e;
-    for (hipEvent_t *ev : {&w.ev_fork, &w.ev_ctl, &w.ev_next, &w.ev_join, &w.ev_solved, &w.ev_tail})
+    for (hipEvent_t *ev : {&w.ev_fork, &w.ev_next, &w.ev_join, &w.ev_solved, &w.ev_tail,
+                           &w.ev_pan[0], &w.ev_pan[1], &w.ev_pan[2], &w.ev_pan[3], &w.ev_near,
+                           &w.ev_far})
       if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
     w.ready = true;
@@ -361,8 +368,21 @@ struct Run {
     int c0, c1, first;  // columns [c0, c1) have taken the row movements of events < first
   };
   std::vector<Block> blocks;
-  const int *cur_list = nullptr;  // swap list of the panel launched last, and its event
-  int cur_event = -1;
+  // A panel launch: its block, swap list (and archive event, deferred mode), done token (its
+  // done_seq, which also picks its host-visible slot and end event) and whether it was queued on
+  // speculation (before the panel ahead of it finished).
+  struct Launch {
+    bool valid = false;
+    int pb = 0, pe = 0;
+    const int *list = nullptr;
+    int event = -1;
+    unsigned long long seq = 0;
+    bool spec = false;
+  };
+  Launch launched;      // the last panel launch
+  int launches = 0;     // panel launches so far (they take the swap lists in turn)
+  // Predicate of the launches being queued: set while a speculative step is queued.
+  Pred pred{nullptr, 0};
   hipStream_t ps = nullptr, us = nullptr;  // internal panel / update streams
 
   // Opt-in launch timing: an event pair around each launch, on the launch's own stream.
@@ -417,10 +437,8 @@ struct Run {
   // Tall panel: one column step at a time (column argmax, whole-row interchange, multipliers and
   // rank-1 update), every launch predicated on the control block; rows are interchanged across
   // all columns at once, so its swap list is the identity.  Runs in order on `st`.
-  hipError_t tall_panel(int pb, int pe, int positioned, int parity, hipStream_t st) {
-    int *list = w.d_lists + parity * kListStride;
-    cur_list = list;
-    cur_event = -1;
+  hipError_t tall_panel(int pb, int pe, int positioned, hipStream_t st) {
+    int *list = w.d_lists + (launches++ % kLists) * kListStride;
     hipError_t e = profile ? mark(t_panel, st, 0.0) : hipSuccess;
     if (e == hipSuccess) e = launch_tall_begin(w.d_ctl, pb, pe, st);
     for (int k = pb; e == hipSuccess && k < pe; ++k) {
@@ -430,20 +448,24 @@ struct Run {
       }
       if (e == hipSuccess) e = launch_tall_update(a, lda, m, k, pe, w.d_ctl, st);
     }
-    expect_done = ++w.done_seq;
-    if (e == hipSuccess) e = launch_tall_finish(w.d_ctl, list, nb, pb, w.d_done, expect_done, st);
+    const unsigned long long seq = ++w.done_seq;
+    if (e == hipSuccess) e = launch_tall_finish(w.d_ctl, list, nb, pb, done_slot(seq), seq, st);
     if (e == hipSuccess && profile) e = close(t_panel, st);
     if (e != hipSuccess) return e;
     ++panels;
-    return single ? hipSuccess : hipEventRecord(w.ev_ctl, st);
+    launched = Launch{true, pb, pe, list, -1, seq, false};
+    return single ? hipSuccess : hipEventRecord(w.ev_pan[seq % kDoneSlots], st);
+  }
+  PanelDone *done_slot(unsigned long long seq) const {
+    return w.d_done ? w.d_done + seq % kDoneSlots : nullptr;
   }
 
-  hipError_t panel(int pb, int pe, int positioned, int parity, hipStream_t st) {
+  hipError_t panel(int pb, int pe, int positioned, hipStream_t st) {
     int groups = 0, rpl = 0;
     const bool lag = panel_kernel != DLU_PANEL_MOVE &&
                      lag_shape_for(m - pb, nb, sizeof(T), forced_rpl, max_groups, groups, rpl);
     if (!lag && panel_kernel == DLU_PANEL_LAG) return hipErrorInvalidValue;
-    if (!lag && !panel_shape(pb, groups, rpl)) return tall_panel(pb, pe, positioned, parity, st);
+    if (!lag && !panel_shape(pb, groups, rpl)) return tall_panel(pb, pe, positioned, st);
     PanelParams<T> p{};
     p.a = a;
     p.lda = lda;
@@ -469,44 +491,44 @@ struct Run {
     p.rowk = w.d_rowk;
     if (defer && events >= w.archive_cap) return hipErrorInvalidValue;  // (one list per panel)
     p.swap_list = defer ? w.d_archive + static_cast<size_t>(events) * kListStride
-                        : w.d_lists + parity * kListStride;
-    cur_list = p.swap_list;
-    cur_event = defer ? events++ : -1;
+                        : w.d_lists + (launches % kLists) * kListStride;
+    ++launches;
+    const int event = defer ? events++ : -1;
     p.spin_limit = spin;
     p.phase_ticks = profile ? w.d_phase : nullptr;
-    p.done = w.d_done;
     p.done_seq = ++w.done_seq;
-    expect_done = p.done_seq;
+    p.done = done_slot(p.done_seq);
+    p.pred = pred;
     hipError_t e = profile ? mark(t_panel, st, 0.0) : hipSuccess;
     if (e == hipSuccess) e = lag ? launch_panel_lag(p, nb, rpl, st) : launch_panel(p, nb, rpl, st);
     if (e == hipSuccess && profile) e = close(t_panel, st);
     if (e != hipSuccess) return e;
     ++panels;
-    return single ? hipSuccess : hipEventRecord(w.ev_ctl, st);
+    launched = Launch{true, pb, pe, p.swap_list, event, p.done_seq, pred.go != nullptr};
+    return single ? hipSuccess : hipEventRecord(w.ev_pan[p.done_seq % kDoneSlots], st);
   }
 
-  // The last panel's control block: spin on the sequence number the panel kernel stores into
-  // host memory as its last act.  Should the panel have finished without publishing (its event
-  // completes first), the device copy is read instead.
-  unsigned long long expect_done = 0;
-  hipError_t wait_panel(Control &c) {
+  // A panel's control block: spin on the sequence number the panel kernel stores into its host
+  // slot as its last act.  A launch whose end event completes without that store has failed.
+  hipError_t wait_panel(const Launch &L, Control &c) {
     if (single || w.h_done == nullptr) {  // no pinned copy: read the device's
       hipError_t e = hipMemcpyAsync(&c, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost, s);
       return e == hipSuccess ? hipStreamSynchronize(s) : e;
     }
+    PanelDone &d = w.h_done[L.seq % kDoneSlots];
     for (unsigned it = 1;; ++it) {
-      if (__atomic_load_n(&w.h_done->seq, __ATOMIC_ACQUIRE) == expect_done) {
-        c = w.h_done->c;
+      if (__atomic_load_n(&d.seq, __ATOMIC_ACQUIRE) == L.seq) {
+        c = d.c;
         return hipSuccess;
       }
       if ((it & 255) == 0) {
-        const hipError_t q = hipEventQuery(w.ev_ctl);
+        const hipError_t q = hipEventQuery(w.ev_pan[L.seq % kDoneSlots]);
         if (q == hipSuccess) {
-          if (__atomic_load_n(&w.h_done->seq, __ATOMIC_ACQUIRE) == expect_done) {
-            c = w.h_done->c;
+          if (__atomic_load_n(&d.seq, __ATOMIC_ACQUIRE) == L.seq) {
+            c = d.c;
             return hipSuccess;
           }
-          return hipMemcpy(&c, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost);
+          return hipErrorLaunchFailure;
         }
         if (q != hipErrorNotReady) return q;
       }
@@ -527,7 +549,7 @@ struct Run {
     hipError_t e = profile ? mark(tv, st, flops, bytes) : hipSuccess;
     if (e == hipSuccess)
       e = launch_gemm_sub<T>(at(a, lda, r0, c0), lda, at(a, lda, r0, kc), lda, at(a, lda, kr, c0),
-                             lda, M, N, K, st);
+                             lda, M, N, K, st, -1, pred);
     if (e == hipSuccess && profile) e = close(tv, st);
     return e;
   }
@@ -537,7 +559,7 @@ struct Run {
   hipError_t solve(int pb, int wdt, int c0, int c1, const int *list, hipStream_t st) {
     if (c1 <= c0 || wdt <= 0) return hipSuccess;
     hipError_t e = profile ? mark(t_solve, st, 0.0) : hipSuccess;
-    if (e == hipSuccess) e = launch_swap_trsm(a, lda, pb, wdt, c0, c1, list, nb, st);
+    if (e == hipSuccess) e = launch_swap_trsm(a, lda, pb, wdt, c0, c1, list, nb, st, pred);
     if (e == hipSuccess && profile) e = close(t_solve, st);
     return e;
   }
@@ -577,7 +599,7 @@ struct Run {
   hipError_t pair_prepare(int b0, const int *l0, const int *l1, int c0, int c1, hipStream_t st) {
     const int b1 = b0 + nb;
     hipError_t e = solve(b0, nb, c0, c1, l0, st);
-    if (e == hipSuccess) e = launch_row_interchange(a, lda, c0, c1, l1, nb, st);
+    if (e == hipSuccess) e = launch_row_interchange(a, lda, c0, c1, l1, nb, st, pred);
     if (e == hipSuccess) e = gemm(b1, c0, b0, b0, nb, c1 - c0, nb, st);
     if (e == hipSuccess) e = solve(b1, nb, c0, c1, nullptr, st);
     return e;
@@ -716,117 +738,140 @@ struct Run {
   // The next panel, queued on `ps` behind everything `us` holds so far (the next panel's block),
   // as soon as that block's update is queued: the far work of the step is queued after it, so
   // the host's launch calls for the far work are off the panel chain.
-  hipError_t launch_next(int pb_next, int pe_next, int positioned, int parity) {
-    if (single || tall(pb_next))
-      return panel(pb_next, pe_next, positioned, (parity + 1) % kLists, us);
+  hipError_t launch_next(int pb_next, int pe_next, int positioned) {
+    if (single || tall(pb_next)) return panel(pb_next, pe_next, positioned, us);
     hipError_t e = hipEventRecord(w.ev_next, us);
     if (e == hipSuccess) e = hipStreamWaitEvent(ps, w.ev_next, 0);
-    if (e == hipSuccess) e = panel(pb_next, pe_next, positioned, (parity + 1) % kLists, ps);
+    if (e == hipSuccess) e = panel(pb_next, pe_next, positioned, ps);
     return e;
+  }
+
+  // What follows a panel [pb, pe) that eliminated all its columns.  Pair2: it closes a pair (its
+  // near work is P0's interchange by its list and the pair's K = 2 nb update of the next block,
+  // its far work the pair's left half).  Pair1: it opens one (the next block alone now; the
+  // previous pair's right half follows).  Single: the next block, then every far column.
+  // Whole: no look-ahead (last panel, a tall next panel, look-ahead off): everything, then the
+  // next panel.
+  enum class Step { Whole, Pair2, Pair1, Single };
+  Step step_after(int pb, int pe, bool lookahead) const {
+    const int pe2 = std::min(pe + nb, steps);
+    if (pe - pb != nb || pe >= steps || !lookahead || tall(pe)) return Step::Whole;
+    if (pair.open) return Step::Pair2;
+    if (pe2 < n && m - pe2 >= merge_rows) return Step::Pair1;
+    return Step::Single;
+  }
+  // The work the next panel waits for.
+  hipError_t near(Step st, const Launch &L, hipStream_t q) {
+    const int pe2 = std::min(L.pe + nb, steps);
+    if (st == Step::Pair2) {
+      hipError_t e = launch_row_interchange(a, lda, pair.b0, L.pb, L.list, nb, q, pred);
+      return e == hipSuccess ? pair_update(pair.b0, pair.l0, L.list, L.pe, pe2, q) : e;
+    }
+    return update(L.pb, nb, L.pe, pe2, L.list, q);
+  }
+  // The rest of the step, on `us`, once its near work is done.
+  hipError_t far(Step st, const Launch &L) {
+    const int pb = L.pb, pe = L.pe, pe2 = std::min(pe + nb, steps);
+    hipError_t e;
+    if (st == Step::Pair2) {
+      // The left half of the far columns (it holds the block after next); the right half is
+      // queued after the next step's near work.
+      const int b0 = pair.b0;
+      pair.open = false;
+      if ((e = queue_rest(0)) != hipSuccess) return e;  // (none pending: a pair just opened)
+      if ((e = wait_tail(us)) != hipSuccess) return e;
+      const int half = std::max(nb, (n - pe2) / 2 / nb * nb);
+      const int cm = std::min(n, pe2 + half);
+      if ((e = pair_update(b0, pair.l0, L.list, pe2, cm, us)) != hipSuccess) return e;
+      rest = Rest{true, b0, cm, pair.l0, L.list};
+      if (defer) {  // P0's columns took P1's movement in the near work
+        blocks.push_back(Block{b0, pb, L.event + 1});
+        blocks.push_back(Block{pb, pe, L.event + 1});
+      }
+      return hipSuccess;
+    }
+    if (st == Step::Pair1) {
+      // Only the next panel's columns now (and the previous pair's right half, which the next
+      // panel's block does not touch).
+      if ((e = queue_rest(pe2 - pe)) != hipSuccess) return e;
+      pair.open = true;
+      pair.b0 = pb;
+      pair.fe = pe2;
+      pair.l0 = L.list;
+      pair.e0 = L.event;
+      return hipSuccess;
+    }
+    // The far columns while the next panel runs (their last columns on the panel stream behind
+    // it when the GEMM outlasts it).
+    if ((e = queue_rest(0)) != hipSuccess) return e;
+    if ((e = wait_tail(us)) != hipSuccess) return e;
+    if ((e = solve(pb, nb, pe2, n, L.list, us)) != hipSuccess) return e;
+    const int cb = tail_split(m - pb - nb, nb, pe2, pe2 - pe);
+    if (cb < n) {
+      if ((e = hipEventRecord(w.ev_solved, us)) != hipSuccess) return e;
+      tail = Tail{true, false, pb + nb, pb, nb, cb};
+    }
+    if ((e = gemm(pb + nb, pe2, pb, pb, m - pb - nb, cb - pe2, nb, us)) != hipSuccess) return e;
+    return left(pb, pb + nb, L.event, L.list, us);
   }
 
   // Runs on the panel stream `ps` and the update stream `us`, forked from and joined back to the
   // caller's stream.  Per panel the host reads the control block once to learn where the panel
   // stopped; every pivot decision inside the panel is taken on the device.
+  //
+  // Speculation (two streams without CU partitions or panel-stream GEMM tails): every near work
+  // goes on `ps` between the panels, and while panel j runs the host already queues panel j's
+  // near work and panel j+1 behind it, on the guess that panel j eliminates all its columns.
+  // Those launches are predicated on panel j's token (Control::go) and do nothing when the guess
+  // fails, so the panel chain runs without a host round trip or a cross-stream wait between
+  // panels.  `ev_far` (everything `us` holds) orders each near work after the far work before it;
+  // `ev_near` orders the far work of a step after its near work.
   hipError_t factor(int *info, bool lookahead) {
     hipError_t e;
     if (single) lookahead = false;
+    const bool speculate = lookahead && panel_cus == 0 && forced_split <= 0;
     if (!single && ((e = hipEventRecord(w.ev_fork, s)) != hipSuccess ||
                     (e = hipStreamWaitEvent(ps, w.ev_fork, 0)) != hipSuccess ||
-                    (e = hipStreamWaitEvent(us, w.ev_fork, 0)) != hipSuccess))
+                    (e = hipStreamWaitEvent(us, w.ev_fork, 0)) != hipSuccess ||
+                    (e = hipEventRecord(w.ev_far, us)) != hipSuccess))
       return e;
-    int pb = 0, parity = 0;
-    int pe = std::min(pb + nb, steps);
-    if ((e = panel(pb, pe, 0, parity, tall(pb) ? us : ps)) != hipSuccess) return e;
+    if ((e = panel(0, std::min(nb, steps), 0, tall(0) ? us : ps)) != hipSuccess) return e;
+    Launch cur = launched, nxt;  // the panel awaited; the one queued behind it on speculation
     for (;;) {
       Control c;
-      if ((e = wait_panel(c)) != hipSuccess) return e;
+      if ((e = wait_panel(cur, c)) != hipSuccess) return e;
       if (c.error != 0) return hipErrorLaunchFailure;
+      const int pb = cur.pb, pe = cur.pe;
       const int stop = c.stop;
       const int wdt = stop - pb;
-      const int *list = cur_list;  // the panel just waited for
-      const int ev = cur_event;
       const int pe2 = std::min(pe + nb, steps);
-      const bool full = !c.deferred && !c.zero_column && wdt == nb && pe - pb == nb;
-      if (!single && (e = hipStreamWaitEvent(us, w.ev_ctl, 0)) != hipSuccess) return e;
-      // A tall next panel interchanges whole rows as it goes: nothing may run beside it.
-      const bool ahead = lookahead && !tall(pe);
-      const bool second = pair.open && full && pe < steps && ahead;
-      if (!second && (e = close_pair(us)) != hipSuccess) return e;
+      if (!single && (e = hipStreamWaitEvent(us, w.ev_pan[cur.seq % kDoneSlots], 0)) != hipSuccess)
+        return e;
+      const bool whole = !c.deferred && !c.zero_column && wdt == pe - pb;
+      const Step st = whole ? step_after(pb, pe, lookahead) : Step::Whole;
+      if (nxt.valid && !(whole && st != Step::Whole)) {
+        // The guess failed: the near work and the panel queued behind `cur` did nothing.  Nothing
+        // else may touch the control block before they are through.
+        if ((e = hipStreamWaitEvent(us, w.ev_pan[nxt.seq % kDoneSlots], 0)) != hipSuccess) return e;
+        nxt = Launch{};
+      }
+      if (st != Step::Pair2) {
+        if ((e = close_pair(us)) != hipSuccess) return e;
+      }
       if (c.deferred) {
         if ((e = queue_rest(0)) != hipSuccess) return e;
         if ((e = wait_tail(us)) != hipSuccess) return e;
         *info = c.stop;
         break;
       }
-      if (!c.zero_column) {
-        if (second) {
-          // Second panel of a pair: P0's L columns take P1's interchange, then the next panel's
-          // block, then the left half of the far columns (it holds the block after next); the
-          // right half is queued after the next step's near update.
-          const int b0 = pair.b0;
-          pair.open = false;
-          if ((e = queue_rest(0)) != hipSuccess) return e;  // (none pending: a pair just opened)
-          if ((e = wait_tail(us)) != hipSuccess) return e;
-          if ((e = launch_row_interchange(a, lda, b0, pb, list, nb, us)) != hipSuccess) return e;
-          if ((e = pair_update(b0, pair.l0, list, pe, pe2, us)) != hipSuccess) return e;
-          if ((e = launch_next(pe, pe2, 0, parity)) != hipSuccess) return e;
-          const int half = std::max(nb, (n - pe2) / 2 / nb * nb);
-          const int cm = std::min(n, pe2 + half);
-          if ((e = pair_update(b0, pair.l0, list, pe2, cm, us)) != hipSuccess) return e;
-          rest = Rest{true, b0, cm, pair.l0, list};
-          if (defer) {  // P0's columns took P1's movement above
-            blocks.push_back(Block{b0, pb, ev + 1});
-            blocks.push_back(Block{pb, pe, ev + 1});
-          }
-        } else if (pe < steps && ahead && full && pe2 < n && m - pe2 >= merge_rows) {
-          // First panel of a pair: only the next panel's columns now (and the previous pair's
-          // right half, which the next panel's block does not touch).
-          if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
-          if ((e = launch_next(pe, pe2, 0, parity)) != hipSuccess) return e;
-          if ((e = queue_rest(pe2 - pe)) != hipSuccess) return e;
-          pair.open = true;
-          pair.b0 = pb;
-          pair.fe = pe2;
-          pair.l0 = list;
-          pair.e0 = ev;
-        } else if (pe < steps && ahead) {
-          // The next panel's columns first, then the far columns while the next panel runs
-          // (their last columns on the panel stream behind it when the GEMM outlasts it).
-          if ((e = update(pb, wdt, pe, pe2, list, us)) != hipSuccess) return e;
-          if ((e = launch_next(pe, pe2, 0, parity)) != hipSuccess) return e;
-          if ((e = queue_rest(0)) != hipSuccess) return e;
-          if ((e = wait_tail(us)) != hipSuccess) return e;
-          if ((e = solve(pb, wdt, pe2, n, list, us)) != hipSuccess) return e;
-          const int cb = tail_split(m - pb - wdt, wdt, pe2, pe2 - pe);
-          if (cb < n) {
-            if ((e = hipEventRecord(w.ev_solved, us)) != hipSuccess) return e;
-            tail = Tail{true, false, pb + wdt, pb, wdt, cb};
-          }
-          if ((e = gemm(pb + wdt, pe2, pb, pb, m - pb - wdt, cb - pe2, wdt, us)) != hipSuccess)
-            return e;
-          if ((e = left(pb, pb + wdt, ev, list, us)) != hipSuccess) return e;
-        } else {
-          if ((e = queue_rest(0)) != hipSuccess) return e;
-          if ((e = wait_tail(us)) != hipSuccess) return e;
-          if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
-          if ((e = left(pb, pb + wdt, ev, list, us)) != hipSuccess) return e;
-          if (pe >= steps) {
-            *info = steps;
-            break;
-          }
-          if ((e = launch_next(pe, pe2, 0, parity)) != hipSuccess) return e;
-        }
-        pb = pe;
-        pe = pe2;
-      } else {
+      if (c.zero_column) {
         // Zero column at `stop`: bring the far columns up to date with the accepted prefix
         // (delayed_dgetrf.cpp:183-186), search the whole active block, interchange, restart.
         if ((e = queue_rest(0)) != hipSuccess) return e;
         if ((e = wait_tail(us)) != hipSuccess) return e;
         if (wdt > 0) {
-          if ((e = update(pb, wdt, pe, n, list, us)) != hipSuccess) return e;
-          if ((e = left(pb, pb + wdt, ev, list, us)) != hipSuccess) return e;
+          if ((e = update(pb, wdt, pe, n, cur.list, us)) != hipSuccess) return e;
+          if ((e = left(pb, pb + wdt, cur.event, cur.list, us)) != hipSuccess) return e;
         }
         // The fallback interchanges whole rows: every owed left movement goes first.
         if ((e = apply_deferred(us)) != hipSuccess) return e;
@@ -843,19 +888,70 @@ struct Run {
         if ((e = launch_fallback_apply(a, lda, m, n, stop, w.d_ipiv, w.d_jpiv, w.d_ctl, us)) !=
             hipSuccess)
           return e;
-        pb = stop;
-        pe = std::min(pb + nb, steps);
+        const int rb = stop, re = std::min(rb + nb, steps);
         // The restarted panel follows the fallback (nothing runs beside it): on the same stream,
         // or, when the panel stream owns a CU partition sized for panels, there, behind it.
-        if (panel_cus > 0 && !tall(pb)) {
+        if (panel_cus > 0 && !tall(rb)) {
           if ((e = hipEventRecord(w.ev_next, us)) != hipSuccess) return e;
           if ((e = hipStreamWaitEvent(ps, w.ev_next, 0)) != hipSuccess) return e;
-          if ((e = panel(pb, pe, 1, (parity + 1) % kLists, ps)) != hipSuccess) return e;
-        } else if ((e = panel(pb, pe, 1, (parity + 1) % kLists, us)) != hipSuccess) {
+          if ((e = panel(rb, re, 1, ps)) != hipSuccess) return e;
+        } else if ((e = panel(rb, re, 1, us)) != hipSuccess) {
           return e;
         }
+        cur = launched;
+      } else if (st == Step::Whole) {
+        if ((e = queue_rest(0)) != hipSuccess) return e;
+        if ((e = wait_tail(us)) != hipSuccess) return e;
+        if ((e = update(pb, wdt, pe, n, cur.list, us)) != hipSuccess) return e;
+        if ((e = left(pb, pb + wdt, cur.event, cur.list, us)) != hipSuccess) return e;
+        if (pe >= steps) {
+          *info = steps;
+          break;
+        }
+        if ((e = launch_next(pe, pe2, 0)) != hipSuccess) return e;
+        cur = launched;
+      } else if (speculate) {
+        if (!nxt.valid) {  // nothing queued behind `cur` yet: its near work and the next panel
+          // (`cur` may have run on `us`: a restart after a fallback, or a tall first panel)
+          if ((e = hipStreamWaitEvent(ps, w.ev_pan[cur.seq % kDoneSlots], 0)) != hipSuccess) return e;
+          if ((e = hipStreamWaitEvent(ps, w.ev_far, 0)) != hipSuccess) return e;
+          if ((e = near(st, cur, ps)) != hipSuccess) return e;
+          if ((e = hipEventRecord(w.ev_near, ps)) != hipSuccess) return e;
+          if ((e = panel(pe, pe2, 0, ps)) != hipSuccess) return e;
+          nxt = launched;
+        }
+        if ((e = hipStreamWaitEvent(us, w.ev_near, 0)) != hipSuccess) return e;
+        if ((e = far(st, cur)) != hipSuccess) return e;
+        if ((e = hipEventRecord(w.ev_far, us)) != hipSuccess) return e;
+        cur = nxt;
+        nxt = Launch{};
+        // The guess for the panel now running: its near work and the panel after it.  (A tall
+        // panel publishes no token, so nothing is guessed behind one.)
+        const Step sn = step_after(cur.pb, cur.pe, lookahead);
+        if (sn != Step::Whole && !tall(cur.pb)) {
+          pred = Pred{&w.d_ctl->go, cur.seq};
+          e = hipStreamWaitEvent(ps, w.ev_far, 0);
+          if (e == hipSuccess) e = near(sn, cur, ps);
+          if (e == hipSuccess) e = hipEventRecord(w.ev_near, ps);
+          if (e == hipSuccess) e = panel(cur.pe, std::min(cur.pe + nb, steps), 0, ps);
+          pred = Pred{nullptr, 0};
+          if (e != hipSuccess) return e;
+          nxt = launched;
+        }
+      } else {
+        // The next panel's block first (a pair's second panel: after P0's interchange by its
+        // list), then the far columns while the next panel runs.
+        if (st == Step::Pair2) {
+          if ((e = queue_rest(0)) != hipSuccess) return e;
+          if ((e = wait_tail(us)) != hipSuccess) return e;
+        }
+        if ((e = near(st, cur, us)) != hipSuccess) return e;
+        if ((e = launch_next(pe, pe2, 0)) != hipSuccess) return e;
+        const Launch next = launched;
+        if ((e = far(st, cur)) != hipSuccess) return e;
+        cur = next;
       }
-      parity = (parity + 1) % kLists;
+      if (speculate && (e = hipEventRecord(w.ev_far, us)) != hipSuccess) return e;
       if ((e = launch_tail()) != hipSuccess) return e;
     }
     return apply_deferred(us);
@@ -879,7 +975,8 @@ void release_workspace(Workspace &w) {
     if (w.streams[i].panel) hipStreamDestroy(w.streams[i].panel);
     if (w.streams[i].update) hipStreamDestroy(w.streams[i].update);
   }
-  for (hipEvent_t ev : {w.ev_fork, w.ev_ctl, w.ev_next, w.ev_join, w.ev_solved, w.ev_tail})
+  for (hipEvent_t ev : {w.ev_fork, w.ev_next, w.ev_join, w.ev_solved, w.ev_tail, w.ev_pan[0],
+                        w.ev_pan[1], w.ev_pan[2], w.ev_pan[3], w.ev_near, w.ev_far})
     if (ev) hipEventDestroy(ev);
   for (void *p : {static_cast<void *>(w.d_ctl), static_cast<void *>(w.d_ipiv),
                   static_cast<void *>(w.d_jpiv), static_cast<void *>(w.d_margins),
@@ -1110,7 +1207,8 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   // Left interchanges deferred to one pass at the end: the panels' lists stay (one per panel),
   // and each column is permuted once (through LDS: at most kDeferMaxRows rows).
   if (m <= kDeferMaxRows && !run.tall(0) && !(opt && opt->defer_left < 0)) {
-    const int lists = (steps + nb - 1) / nb;
+    // (one more: a panel queued on speculation behind the first zero column takes a list too)
+    const int lists = (steps + nb - 1) / nb + 1;
     run.defer = prepare_defer(w, m, n, lists) == hipSuccess && w.archive_cap >= lists;
   }
   return execute(run, lookahead, info, ipiv, jpiv, margins, rep);

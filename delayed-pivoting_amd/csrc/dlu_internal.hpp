@@ -31,7 +31,21 @@ struct Control {
   unsigned search_ticket;
   unsigned abort_flag;  // set by a panel workgroup whose wait expired; the others leave too
   int col_found;        // column-variant search: rightmost nonzero column so far (-1: none)
+  // The done_seq of the last panel that eliminated all its columns (no zero column, nothing
+  // deferred): the token the launches queued behind it on speculation test.
+  unsigned long long go;
 };
+
+// A launch queued before the panel it follows has finished, on the guess that the panel
+// eliminates all its columns: it does its work only when *go (Control::go) holds that panel's
+// token, and returns at once otherwise.  go == nullptr: unconditional.
+struct Pred {
+  const unsigned long long *go;
+  unsigned long long expect;
+};
+__device__ __forceinline__ bool skipped(const Pred &p) {
+  return p.go != nullptr && *p.go != p.expect;
+}
 
 // A panel's control block as the host sees it: workgroup 0 of the panel kernel copies the
 // control block here (pinned, coherent host memory) as its last act, then stores `seq` with
@@ -121,6 +135,7 @@ struct PanelParams {
   unsigned long long *phase_ticks;  // optional [8] per-phase tick totals (workgroup 0)
   PanelDone *done;                  // host-visible copy of the control block (may be null)
   unsigned long long done_seq;      // value of done->seq once this launch has published
+  Pred pred;                        // a speculative launch: it factors nothing unless pred holds
 };
 
 // Launchers (return hipSuccess or the launch error).  T is float or double; both are
@@ -137,7 +152,7 @@ hipError_t launch_panel_lag(const PanelParams<T> &p, int nb, int rows_per_lane, 
 // U12 = L11^{-1} * (rows after interchange); rows displaced below pb+w are written directly.
 template <typename T>
 hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const int *swap_list,
-                            int nb, hipStream_t s);
+                            int nb, hipStream_t s, Pred pred = Pred{nullptr, 0});
 
 // C(M x N) -= A(M x K) * B(K x N), column-major with leading dimensions ldc, lda, ldb.
 // gemm_uses_large_tiles names the launches the profiled report counts as large (at least 512
@@ -149,7 +164,7 @@ inline bool gemm_uses_large_tiles(int M, int N) {
 // one (0..3: 64x64, 128x128, 128x64, 64x128).
 template <typename T>
 hipError_t launch_gemm_sub(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
-                           int K, hipStream_t s, int config = -1);
+                           int K, hipStream_t s, int config = -1, Pred pred = Pred{nullptr, 0});
 // The configuration the table picks for an M x N x K update.
 int tuned_gemm_config(int M, int N, int K);
 constexpr int kGemmConfigs = 4;
@@ -157,7 +172,7 @@ constexpr int kGemmConfigs = 4;
 // Row interchanges of the last panel applied to columns [c0, c1) (left of the panel).
 template <typename T>
 hipError_t launch_row_interchange(T *a, int lda, int c0, int c1, const int *swap_list, int nb,
-                                  hipStream_t s);
+                                  hipStream_t s, Pred pred = Pred{nullptr, 0});
 
 // Deferred left interchanges (factorizations of at most kDeferMaxRows rows): compose the kept
 // net row movements of `events` panels backward, writing the permutation into snaps[slot] at

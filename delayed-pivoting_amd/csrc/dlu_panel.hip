@@ -183,7 +183,9 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
     }
   }
   if (threadIdx.x == 0) s_fail = 0;
-  const bool deferred = P.ctl->deferred != 0;  // a speculative restart after a deferral
+  // Nothing to factor: a speculative restart after a deferral, or a launch queued behind a panel
+  // that did not eliminate all its columns.
+  const bool deferred = P.ctl->deferred != 0 || skipped(P.pred);
   __syncthreads();
   if (deferred) {
     if (g == 0 && threadIdx.x == 0) {
@@ -584,6 +586,7 @@ __global__ __launch_bounds__(NB * 4) void panel_kernel(PanelParams<T> P) {
           }
           P.ctl->stop = stop;
           P.ctl->zero_column = zero_col;
+          if (stop == P.pe && zero_col == 0) P.ctl->go = P.done_seq;
           P.ctl->near_ties += near_ties;
           P.ctl->exact_ties += exact_ties;
           if (min_margin < P.ctl->min_margin) P.ctl->min_margin = min_margin;

@@ -343,7 +343,9 @@ __global__ __launch_bounds__(256) void fallback_apply_kernel(T *a, int lda, int 
 template <typename T, int NB>
 __global__ __launch_bounds__(256) void row_interchange_kernel(T *__restrict__ a, int lda,
                                                              int c0, int c1,
-                                                             const int *__restrict__ swap_list) {
+                                                             const int *__restrict__ swap_list,
+                                                             Pred pred) {
+  if (skipped(pred)) return;
   constexpr int PER = (2 * NB + 63) / 64;
   const int cnt = swap_list[4 * NB];
   const int lane = threadIdx.x & 63;
@@ -521,13 +523,13 @@ hipError_t launch_left_gather(T *a, int lda, int m, int ncols, const int *col_sl
 
 template <typename T>
 hipError_t launch_row_interchange(T *a, int lda, int c0, int c1, const int *swap_list, int nb,
-                                  hipStream_t s) {
+                                  hipStream_t s, Pred pred) {
   if (c1 <= c0) return hipSuccess;
   const dim3 grid((c1 - c0 + 3) / 4), block(256);
   switch (nb) {
-    case 64: hipLaunchKernelGGL((row_interchange_kernel<T, 64>), grid, block, 0, s, a, lda, c0, c1, swap_list); break;
-    case 128: hipLaunchKernelGGL((row_interchange_kernel<T, 128>), grid, block, 0, s, a, lda, c0, c1, swap_list); break;
-    case 256: hipLaunchKernelGGL((row_interchange_kernel<T, 256>), grid, block, 0, s, a, lda, c0, c1, swap_list); break;
+    case 64: hipLaunchKernelGGL((row_interchange_kernel<T, 64>), grid, block, 0, s, a, lda, c0, c1, swap_list, pred); break;
+    case 128: hipLaunchKernelGGL((row_interchange_kernel<T, 128>), grid, block, 0, s, a, lda, c0, c1, swap_list, pred); break;
+    case 256: hipLaunchKernelGGL((row_interchange_kernel<T, 256>), grid, block, 0, s, a, lda, c0, c1, swap_list, pred); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -538,7 +540,8 @@ hipError_t launch_row_interchange(T *a, int lda, int c0, int c1, const int *swap
                                               unsigned long long *, Control *, hipStream_t);    \
   template hipError_t launch_fallback_apply<T>(T *, int, int, int, int, int *, int *, Control *, \
                                                hipStream_t);                                     \
-  template hipError_t launch_row_interchange<T>(T *, int, int, int, const int *, int, hipStream_t); \
+  template hipError_t launch_row_interchange<T>(T *, int, int, int, const int *, int, hipStream_t, \
+                                                Pred);                                          \
   template hipError_t launch_left_gather<T>(T *, int, int, int, const int *, const int *,         \
                                             const int *, hipStream_t);                            \
   template hipError_t launch_column_search<T>(const T *, int, int, int, int, Control *,          \

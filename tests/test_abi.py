@@ -61,7 +61,8 @@ def test_null_pointers():
 def test_invalid_options_rejected_before_device_work():
     # an unsupported panel width, rows per lane or fallback variant is rejected (info = -1)
     # before any device call, so this runs without a GPU
-    for opts in ({"nb": 96}, {"rows_per_lane": 3}, {"fallback": 2}, {"fallback": -1}):
+    for opts in ({"nb": 96}, {"rows_per_lane": 5}, {"rows_per_lane": -1}, {"fallback": 2},
+                 {"fallback": -1}):
         st, ipiv, jpiv, inf, rep, _ = dlu.getrf_device(4, 4, 1234, 4, options=opts)
         assert st == dlu.Status.INVALID_ARGUMENT and inf == -1, opts
 
