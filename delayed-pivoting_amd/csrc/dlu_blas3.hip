@@ -33,12 +33,12 @@ namespace {
 // Global loads are lane-contiguous (consecutive lanes read consecutive rows of A, consecutive k
 // of B), 16 bytes per lane when the operands allow it.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int BM, int BN, int BK, bool VEC, int OCC>
+template <typename T, int BM, int BN, int BK, bool VEC, int OCC, bool AMAX>
 __global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, int ldc,
                                                          const T *__restrict__ A, int lda,
                                                          const T *__restrict__ B, int ldb,
                                                          int M, int N, int K, int tiles_m,
-                                                         int tiles_n, Pred pred) {
+                                                         int tiles_n, Pred pred, AmaxOut amax) {
   if (skipped(pred)) return;
   typedef Mfma<T> MF;
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -206,6 +206,34 @@ __global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, i
         const int col = ccol(j, r);
         if (col < N) C[static_cast<size_t>(row) + static_cast<size_t>(col) * ldc] = acc[i][j][r];
       }
+  }
+  if (AMAX) {  // the full-pivot argmax of this tile of the result
+    double bv = 0.0;  // a zero entry never displaces the initial candidate
+    unsigned long long bo = 0ull;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int row = crow_base + 16 * i;
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int col = ccol(j, r);
+          const double v = (row < M && col < N) ? fabs(static_cast<double>(acc[i][j][r])) : 0.0;
+          const unsigned long long o =
+              static_cast<unsigned long long>(col + amax.col_shift) *
+                  static_cast<unsigned long long>(amax.rows) +
+              static_cast<unsigned long long>(row);
+          if (better(v, o, bv, bo)) {
+            bv = v;
+            bo = o;
+          }
+        }
+    }
+    block_best(bv, bo);
+    if (threadIdx.x == 0) {
+      amax.v[blockIdx.x] = bv;
+      amax.o[blockIdx.x] = bo;
+    }
   }
 }
 
@@ -381,19 +409,27 @@ __global__ __launch_bounds__(64) void swap_trsm_kernel(T *__restrict__ a, int ld
 
 template <typename T, int BM, int BN, int BK, int OCC>
 hipError_t launch_gemm_tile(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
-                            int K, hipStream_t s, Pred pred) {
+                            int K, hipStream_t s, Pred pred, AmaxOut amax) {
   const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   // 16-byte loads: 16-byte aligned origins, leading dimensions, M and K multiples of the width.
   constexpr int vw = 16 / static_cast<int>(sizeof(T));
   const bool vec = (lda % vw == 0) && (ldb % vw == 0) && (M % vw == 0) && (K % vw == 0) &&
                    (reinterpret_cast<uintptr_t>(a) % 16 == 0) &&
                    (reinterpret_cast<uintptr_t>(b) % 16 == 0);
-  if (vec)
-    hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK, true, OCC>), dim3(tm * tn), dim3(256), 0,
-                       s, c, ldc, a, lda, b, ldb, M, N, K, tm, tn, pred);
-  else
-    hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK, false, OCC>), dim3(tm * tn), dim3(256), 0,
-                       s, c, ldc, a, lda, b, ldb, M, N, K, tm, tn, pred);
+#define DLU_GEMM_LAUNCH(V, A)                                                                    \
+  hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK, V, OCC, A>), dim3(tm * tn), dim3(256), 0, s, \
+                     c, ldc, a, lda, b, ldb, M, N, K, tm, tn, pred, amax)
+  if (amax.v != nullptr) {
+    if (vec)
+      DLU_GEMM_LAUNCH(true, true);
+    else
+      DLU_GEMM_LAUNCH(false, true);
+  } else if (vec) {
+    DLU_GEMM_LAUNCH(true, false);
+  } else {
+    DLU_GEMM_LAUNCH(false, false);
+  }
+#undef DLU_GEMM_LAUNCH
   return hipGetLastError();
 }
 
@@ -404,17 +440,25 @@ namespace {
 // The tile configurations of the trailing-update GEMM (workgroup tile, workgroups per CU).
 template <typename T>
 hipError_t launch_gemm_config(int config, T *c, int ldc, const T *a, int lda, const T *b,
-                              int ldb, int M, int N, int K, hipStream_t s, Pred pred) {
+                              int ldb, int M, int N, int K, hipStream_t s, Pred pred,
+                              AmaxOut amax) {
   switch (config) {
-    case 0: return launch_gemm_tile<T, 64, 64, 16, 4>(c, ldc, a, lda, b, ldb, M, N, K, s, pred);
-    case 1: return launch_gemm_tile<T, 128, 128, 16, 2>(c, ldc, a, lda, b, ldb, M, N, K, s, pred);
-    case 2: return launch_gemm_tile<T, 128, 64, 16, 3>(c, ldc, a, lda, b, ldb, M, N, K, s, pred);
-    case 3: return launch_gemm_tile<T, 64, 128, 16, 3>(c, ldc, a, lda, b, ldb, M, N, K, s, pred);
+    case 0: return launch_gemm_tile<T, 64, 64, 16, 4>(c, ldc, a, lda, b, ldb, M, N, K, s, pred, amax);
+    case 1:
+      return launch_gemm_tile<T, 128, 128, 16, 2>(c, ldc, a, lda, b, ldb, M, N, K, s, pred, amax);
+    case 2: return launch_gemm_tile<T, 128, 64, 16, 3>(c, ldc, a, lda, b, ldb, M, N, K, s, pred, amax);
+    case 3: return launch_gemm_tile<T, 64, 128, 16, 3>(c, ldc, a, lda, b, ldb, M, N, K, s, pred, amax);
     default: return hipErrorInvalidValue;
   }
 }
 
 }  // namespace
+
+int gemm_workgroups(int config, int M, int N) {
+  static const int bm[kGemmConfigs] = {64, 128, 128, 64}, bn[kGemmConfigs] = {64, 128, 64, 128};
+  if (config < 0 || config >= kGemmConfigs) return 0;
+  return ((M + bm[config] - 1) / bm[config]) * ((N + bn[config] - 1) / bn[config]);
+}
 
 int tuned_gemm_config(int M, int N, int K) {
   const long long tiles = static_cast<long long>((M + 127) / 128) * ((N + 127) / 128);
@@ -425,12 +469,12 @@ int tuned_gemm_config(int M, int N, int K) {
 
 template <typename T>
 hipError_t launch_gemm_sub(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
-                           int K, hipStream_t s, int config, Pred pred) {
+                           int K, hipStream_t s, int config, Pred pred, AmaxOut amax) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
   // The tile configuration comes from the table the autotuner wrote (by depth and launch size)
   // unless the caller names one.
   return launch_gemm_config<T>(config >= 0 ? config : tuned_gemm_config(M, N, K), c, ldc, a, lda,
-                               b, ldb, M, N, K, s, pred);
+                               b, ldb, M, N, K, s, pred, amax);
 }
 
 template <typename T>
@@ -465,7 +509,7 @@ hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const 
 
 #define DLU_BLAS3_INSTANTIATE(T)                                                                 \
   template hipError_t launch_gemm_sub<T>(T *, int, const T *, int, const T *, int, int, int, int, \
-                                         hipStream_t, int, Pred);                                \
+                                         hipStream_t, int, Pred, AmaxOut);                       \
   template hipError_t launch_swap_trsm<T>(T *, int, int, int, int, int, const int *, int,        \
                                           hipStream_t, Pred);
 DLU_BLAS3_INSTANTIATE(float)

@@ -69,6 +69,8 @@ struct Workspace {
   int *d_lists = nullptr;
   double *d_pv = nullptr;
   unsigned long long *d_po = nullptr;
+  double *d_gpv = nullptr;               // an update's argmax partials (kAmaxPartials each)
+  unsigned long long *d_gpo = nullptr;
   unsigned long long *d_phase = nullptr;
   unsigned seq = 0;
   // Deferred left interchanges: every panel's swap list kept (archive), the composed
@@ -257,6 +259,9 @@ hipError_t prepare(Workspace &w, int device, int steps) {
     if ((e = hipMalloc(&w.d_lists, kLists * kListStride * sizeof(int))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_pv, kSearchPartials * sizeof(double))) != hipSuccess) return e;
     if ((e = hipMalloc(&w.d_po, kSearchPartials * sizeof(unsigned long long))) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_gpv, kAmaxPartials * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMalloc(&w.d_gpo, kAmaxPartials * sizeof(unsigned long long))) != hipSuccess)
+      return e;
     if ((e = hipMalloc(&w.d_phase, 8 * sizeof(unsigned long long))) != hipSuccess) return e;
     for (hipEvent_t *ev : {&w.ev_fork, &w.ev_next, &w.ev_join, &w.ev_solved, &w.ev_tail,
                            &w.ev_pan[0], &w.ev_pan[1], &w.ev_pan[2], &w.ev_pan[3], &w.ev_near,
@@ -539,7 +544,8 @@ struct Run {
   // Interchange + U block row + trailing update of columns [c0, c1) for the panel rows
   // [pb, pb+w), updating rows [pb+w, m).
   // A(r0:r0+M, c0:c0+N) -= A(r0:r0+M, kc:kc+K) * A(kr:kr+K, c0:c0+N).
-  hipError_t gemm(int r0, int c0, int kc, int kr, int M, int N, int K, hipStream_t st) {
+  hipError_t gemm(int r0, int c0, int kc, int kr, int M, int N, int K, hipStream_t st,
+                  AmaxOut amax = AmaxOut{nullptr, nullptr, 0, 0}) {
     if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
     const double flops = 2.0 * M * static_cast<double>(N) * K;
     // C read and written once, A and B read once: the least memory traffic of the launch.
@@ -549,7 +555,7 @@ struct Run {
     hipError_t e = profile ? mark(tv, st, flops, bytes) : hipSuccess;
     if (e == hipSuccess)
       e = launch_gemm_sub<T>(at(a, lda, r0, c0), lda, at(a, lda, r0, kc), lda, at(a, lda, kr, c0),
-                             lda, M, N, K, st, -1, pred);
+                             lda, M, N, K, st, -1, pred, amax);
     if (e == hipSuccess && profile) e = close(tv, st);
     return e;
   }
@@ -869,20 +875,38 @@ struct Run {
         // (delayed_dgetrf.cpp:183-186), search the whole active block, interchange, restart.
         if ((e = queue_rest(0)) != hipSuccess) return e;
         if ((e = wait_tail(us)) != hipSuccess) return e;
+        // The prefix's update of the far columns [pe, n) is the last write to that part of the
+        // active block: its GEMM leaves the block's argmax there as per-workgroup partials, and
+        // the search reads only the panel's own columns [stop, pe) (H4: materialize + search in
+        // one pass).
+        int ext = 0;
         if (wdt > 0) {
-          if ((e = update(pb, wdt, pe, n, cur.list, us)) != hipSuccess) return e;
+          const int gm = m - stop, gn = n - pe;
+          const int wgs = gn > 0 && gm > 0 ? gemm_workgroups(tuned_gemm_config(gm, gn, wdt), gm, gn) : 0;
+          if (variant == DLU_FALLBACK_FULL && w.d_gpv != nullptr && wgs > 0 &&
+              wgs <= kAmaxPartials) {
+            if ((e = solve(pb, wdt, pe, n, cur.list, us)) != hipSuccess) return e;
+            if ((e = gemm(stop, pe, pb, pb, gm, gn, wdt, us,
+                          AmaxOut{w.d_gpv, w.d_gpo, pe - stop, gm})) != hipSuccess)
+              return e;
+            ext = wgs;
+          } else if ((e = update(pb, wdt, pe, n, cur.list, us)) != hipSuccess) {
+            return e;
+          }
           if ((e = left(pb, pb + wdt, cur.event, cur.list, us)) != hipSuccess) return e;
         }
+        const int scan_end = ext > 0 ? pe : n;  // columns the search itself reads
         // The fallback interchanges whole rows: every owed left movement goes first.
         if ((e = apply_deferred(us)) != hipSuccess) return e;
         if (profile &&
             (e = mark(t_search, us, 0.0,
-                      sizeof(T) * static_cast<double>(m - stop) * static_cast<double>(n - stop))) !=
-                hipSuccess)
+                      sizeof(T) * static_cast<double>(m - stop) *
+                          static_cast<double>(scan_end - stop))) != hipSuccess)
           return e;
         e = variant == DLU_FALLBACK_COLUMN
                 ? launch_column_search(a, lda, m, n, stop, w.d_ctl, us)
-                : launch_active_search(a, lda, m, n, stop, w.d_pv, w.d_po, w.d_ctl, us);
+                : launch_active_search(a, lda, m, scan_end, stop, w.d_pv, w.d_po, w.d_ctl, us,
+                                       w.d_gpv, w.d_gpo, ext);
         if (e != hipSuccess) return e;
         if (profile && (e = close(t_search, us)) != hipSuccess) return e;
         if ((e = launch_fallback_apply(a, lda, m, n, stop, w.d_ipiv, w.d_jpiv, w.d_ctl, us)) !=
@@ -983,6 +1007,7 @@ void release_workspace(Workspace &w) {
                   static_cast<void *>(w.d_slots), static_cast<void *>(w.d_rowk),
                   static_cast<void *>(w.d_lists), static_cast<void *>(w.d_pv),
                   static_cast<void *>(w.d_po), static_cast<void *>(w.d_phase),
+                  static_cast<void *>(w.d_gpv), static_cast<void *>(w.d_gpo),
                   static_cast<void *>(w.d_archive), static_cast<void *>(w.d_perm),
                   static_cast<void *>(w.d_snaps), static_cast<void *>(w.d_plan)})
     if (p) hipFree(p);

@@ -106,6 +106,53 @@ struct Mfma<float> {
   static __device__ __forceinline__ int acc_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// Full-pivot search order (delayed_lu/delayed_dgetrf.cpp:32-47): the larger magnitude, then the
+// lower column-major offset in the active block; NaN never wins (both comparisons are false).
+__device__ __forceinline__ bool better(double v, unsigned long long o, double bv,
+                                       unsigned long long bo) {
+  return v > bv || (v == bv && o < bo);
+}
+
+// The best (value, offset) of a 256-thread workgroup, in thread 0.
+__device__ __forceinline__ void block_best(double &v, unsigned long long &o) {
+  __shared__ double sv[4];
+  __shared__ unsigned long long so[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(v, off);
+    const unsigned long long oo = __shfl_xor(o, off);
+    if (better(ov, oo, v, o)) {
+      v = ov;
+      o = oo;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    sv[wave] = v;
+    so[wave] = o;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < 4; ++i)
+      if (better(sv[i], so[i], v, o)) {
+        v = sv[i];
+        o = so[i];
+      }
+  }
+}
+
+// The full-pivot argmax of an update's result, written by the update itself: workgroup b of the
+// GEMM leaves its best (|value|, offset) at v[b], o[b]; offsets are column-major in the active
+// block whose row 0 is the GEMM's row 0 and whose column 0 lies col_shift columns left of the
+// GEMM's column 0, with `rows` rows.
+struct AmaxOut {
+  double *v;
+  unsigned long long *o;
+  int col_shift;
+  int rows;
+};
+constexpr int kAmaxPartials = 1 << 16;  // most GEMM workgroups an update may leave partials for
+
 // A panel's net row movement: [0, 2nb) destination rows, [2nb, 4nb) source rows, [4nb] count;
 // [4nb+1] and [4nb+2] are the panel kernel's append counter and finished-workgroup count (zero
 // between panels).  Two such lists alternate between consecutive panels so the far update of
@@ -164,7 +211,11 @@ inline bool gemm_uses_large_tiles(int M, int N) {
 // one (0..3: 64x64, 128x128, 128x64, 64x128).
 template <typename T>
 hipError_t launch_gemm_sub(T *c, int ldc, const T *a, int lda, const T *b, int ldb, int M, int N,
-                           int K, hipStream_t s, int config = -1, Pred pred = Pred{nullptr, 0});
+                           int K, hipStream_t s, int config = -1, Pred pred = Pred{nullptr, 0},
+                           AmaxOut amax = AmaxOut{nullptr, nullptr, 0, 0});
+// Workgroups of one launch of tile configuration `config` over an M x N result (the partials an
+// argmax epilogue leaves).
+int gemm_workgroups(int config, int M, int N);
 // The configuration the table picks for an M x N x K update.
 int tuned_gemm_config(int M, int N, int K);
 constexpr int kGemmConfigs = 4;
@@ -188,10 +239,14 @@ hipError_t launch_left_gather(T *a, int lda, int m, int ncols, const int *col_sl
 // Identity swap histories and "unreached" margins for a new factorization.
 hipError_t launch_init_steps(int *ipiv, int *jpiv, double *margins, int steps, hipStream_t s);
 
-// Full-pivot search over the active block A(k:m-1, k:n-1); result in ctl->best_*.
+// Full-pivot search over the active block A(k:m-1, k:n-1); result in ctl->best_*.  With
+// ext_count > 0 the columns from n on were searched by an update's argmax epilogue, which left
+// ext_count partials (ext_v, ext_o); they join the reduction.
 template <typename T>
 hipError_t launch_active_search(const T *a, int lda, int m, int n, int k, double *partial_v,
-                                unsigned long long *partial_o, Control *ctl, hipStream_t s);
+                                unsigned long long *partial_o, Control *ctl, hipStream_t s,
+                                const double *ext_v = nullptr,
+                                const unsigned long long *ext_o = nullptr, int ext_count = 0);
 
 // Column-variant fallback search (delayed_getrf_column): the rightmost column j in (k, n) with a
 // nonzero entry in rows [k, m), and its largest |a| (lowest row on ties); result in ctl->best_*

@@ -14,38 +14,6 @@
 namespace dlu {
 namespace {
 
-__device__ __forceinline__ bool better(double v, unsigned long long o, double bv,
-                                       unsigned long long bo) {
-  return v > bv || (v == bv && o < bo);
-}
-
-__device__ __forceinline__ void block_best(double &v, unsigned long long &o) {
-  __shared__ double sv[4];
-  __shared__ unsigned long long so[4];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double ov = __shfl_xor(v, off);
-    const unsigned long long oo = __shfl_xor(o, off);
-    if (better(ov, oo, v, o)) {
-      v = ov;
-      o = oo;
-    }
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-    sv[wave] = v;
-    so[wave] = o;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 1; i < 4; ++i)
-      if (better(sv[i], so[i], v, o)) {
-        v = sv[i];
-        o = so[i];
-      }
-  }
-}
-
 // The active block is cut into items of one column x 512 rows; every wave of the grid takes
 // items in increasing order (item = column * segments + segment), each lane with 8 independent
 // coalesced loads in flight.  A lane therefore meets its elements in increasing column-major
@@ -58,7 +26,10 @@ __global__ __launch_bounds__(256) void active_search_kernel(const T *__restrict_
                                                            int m, int n, int k,
                                                            double *partial_v,
                                                            unsigned long long *partial_o,
-                                                           Control *ctl) {
+                                                           Control *ctl,
+                                                           const double *__restrict__ ext_v,
+                                                           const unsigned long long *__restrict__ ext_o,
+                                                           int ext_count) {
   constexpr int U = 8, SEG = 64 * U;
   const int rows = m - k, cols = n - k;
   const int nseg = (rows + SEG - 1) / SEG;
@@ -86,6 +57,17 @@ __global__ __launch_bounds__(256) void active_search_kernel(const T *__restrict_
       const bool take = v > best;
       bo = take ? base + static_cast<unsigned long long>(r) : bo;
       best = take ? v : best;
+    }
+  }
+  // Partials an update's argmax epilogue left for the columns right of [k, n) (their offsets
+  // continue this block's column-major numbering), shared out over the grid.
+  for (int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x); i < ext_count;
+       i += static_cast<int>(gridDim.x) * 256) {
+    const double v = ext_v[i];
+    const unsigned long long o = ext_o[i];
+    if (better(v, o, best, bo)) {
+      best = v;
+      bo = o;
     }
   }
   block_best(best, bo);
@@ -443,14 +425,16 @@ hipError_t launch_init_steps(int *ipiv, int *jpiv, double *margins, int steps, h
 
 template <typename T>
 hipError_t launch_active_search(const T *a, int lda, int m, int n, int k, double *partial_v,
-                                unsigned long long *partial_o, Control *ctl, hipStream_t s) {
+                                unsigned long long *partial_o, Control *ctl, hipStream_t s,
+                                const double *ext_v, const unsigned long long *ext_o,
+                                int ext_count) {
   // Four waves per workgroup, about four items per wave when the block is large; at most as
   // many workgroups as there are partial slots.
   const long long items = static_cast<long long>(n - k) * ((m - k + 511) / 512);
   const long long want = (items + 15) / 16;
   const int grid = want < 1 ? 1 : (want > 2048 ? 2048 : static_cast<int>(want));
   hipLaunchKernelGGL(active_search_kernel<T>, dim3(grid), dim3(256), 0, s, a, lda, m, n, k,
-                     partial_v, partial_o, ctl);
+                     partial_v, partial_o, ctl, ext_v, ext_o, ext_count);
   return hipGetLastError();
 }
 
@@ -537,7 +521,8 @@ hipError_t launch_row_interchange(T *a, int lda, int c0, int c1, const int *swap
 
 #define DLU_PIVOT_INSTANTIATE(T)                                                                 \
   template hipError_t launch_active_search<T>(const T *, int, int, int, int, double *,          \
-                                              unsigned long long *, Control *, hipStream_t);    \
+                                              unsigned long long *, Control *, hipStream_t,     \
+                                              const double *, const unsigned long long *, int); \
   template hipError_t launch_fallback_apply<T>(T *, int, int, int, int, int *, int *, Control *, \
                                                hipStream_t);                                     \
   template hipError_t launch_row_interchange<T>(T *, int, int, int, const int *, int, hipStream_t, \
