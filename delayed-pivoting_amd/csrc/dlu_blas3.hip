@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, i
 template <typename T, int NB, bool LIST, bool FULL>
 __global__ __launch_bounds__(64) void swap_trsm_kernel(T *__restrict__ a, int lda, int pb, int w,
                                                       int c0, int c1,
-                                                      const int *__restrict__ swap_list,
+                                                      const int *__restrict__ swap_list, int ls,
                                                       Pred pred) {
   if (skipped(pred)) return;
   typedef Mfma<T> MF;
@@ -267,8 +267,9 @@ __global__ __launch_bounds__(64) void swap_trsm_kernel(T *__restrict__ a, int ld
   __shared__ T D[NT][16][17];  // diagonal blocks of L11, then their inverses
   const int l = threadIdx.x;
   const int lr = l & 15;
-  const int cnt = LIST ? swap_list[4 * NB] : w;
-  const int *src = swap_list + 2 * NB;  // read only with a list
+  // The list was written for panels of width ls >= NB (its layout: ls is the stride).
+  const int cnt = LIST ? swap_list[4 * ls] : w;
+  const int *src = swap_list + 2 * ls;  // read only with a list
   const int cs = c0 + blockIdx.x * 16;
   const int ncol = (c1 - cs) < 16 ? (c1 - cs) : 16;
   const T *L = a + static_cast<size_t>(pb) + static_cast<size_t>(pb) * lda;
@@ -482,22 +483,26 @@ hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const 
                             int nb, hipStream_t s, Pred pred) {
   if (c1 <= c0 || w <= 0) return hipSuccess;
   const dim3 grid((c1 - c0 + 15) / 16), block(64);
+  // The solve's tile height follows w (a restarted panel's accepted prefix is often a few rows):
+  // the 64-row kernel serves any list while w <= 64.
+  const int tile = w <= 64 ? 64 : nb;
 #define DLU_TRSM_CASE(NBV)                                                                      \
   case NBV:                                                                                      \
     if (swap_list && w == NBV)                                                                   \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, true, true>), grid, block, 0, s, a, lda, pb,  \
-                         w, c0, c1, swap_list, pred);                                                  \
+                         w, c0, c1, swap_list, nb, pred);                                        \
     else if (swap_list)                                                                          \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, true, false>), grid, block, 0, s, a, lda, pb, \
-                         w, c0, c1, swap_list, pred);                                                  \
+                         w, c0, c1, swap_list, nb, pred);                                        \
     else if (w == NBV)                                                                           \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, false, true>), grid, block, 0, s, a, lda, pb, \
-                         w, c0, c1, swap_list, pred);                                                  \
+                         w, c0, c1, swap_list, nb, pred);                                        \
     else                                                                                         \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, false, false>), grid, block, 0, s, a, lda,    \
-                         pb, w, c0, c1, swap_list, pred);                                              \
+                         pb, w, c0, c1, swap_list, nb, pred);                                    \
     break;
-  switch (nb) {
+  if (nb != 64 && nb != 128 && nb != 256) return hipErrorInvalidValue;
+  switch (tile) {
     DLU_TRSM_CASE(64)
     DLU_TRSM_CASE(128)
     DLU_TRSM_CASE(256)
@@ -506,6 +511,7 @@ hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const 
 #undef DLU_TRSM_CASE
   return hipGetLastError();
 }
+
 
 #define DLU_BLAS3_INSTANTIATE(T)                                                                 \
   template hipError_t launch_gemm_sub<T>(T *, int, const T *, int, const T *, int, int, int, int, \
