@@ -1173,13 +1173,11 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
     *info = -1;
     return DLU_RUNTIME_FAILURE;
   }
-  // CUs reserved for the panel stream.  FP64: none by default (measured: the trailing update
-  // gains more from the whole device than the panels lose by waiting for update workgroups to
-  // drain from a CU).  FP32, whose trailing update is twice as fast, is panel-bound: above 16
-  // panel workgroups (more than 4096 rows), enough CUs for one resident workgroup per panel row
-  // block (at most half the device), which also lets the far GEMM's tail run on them behind each
-  // panel; smaller factorizations keep the whole device, so several of them (serving) overlap.
-  // opt->panel_cus > 0 forces a size (rounded down to whole XCDs), < 0 disables.
+  // CUs reserved for the panel stream: none by default, FP64 and FP32 alike (measured at
+  // n = 16384: the trailing update gains more from the whole device than the panels lose by
+  // waiting for update workgroups to drain from a CU, and without a partition the panel chain
+  // runs speculatively; FP32 70.5 ms unpartitioned against 72.9 ms with one CU per panel
+  // workgroup).  opt->panel_cus > 0 forces a size (rounded down to whole XCDs).
   // A partition is made of whole XCDs: a stream's CU mask fills the XCDs in order (32 mask bits
   // each on MI355X), and a launch's workgroups are dealt round robin over the XCDs its mask
   // touches, so a partly masked XCD would receive as many panel workgroups as a full one and
@@ -1192,8 +1190,6 @@ dlu_status getrf_device(int m, int n, T *d_a, int lda, int *ipiv, int *jpiv, int
   if (!lag0) panel_shape_for(m, nb, forced_rpl, std::min(kMaxPanelGroups, cus), groups, rpl);
   const int xcd = cus % kCusPerXcd == 0 ? kCusPerXcd : cus;  // unknown layout: no partition
   int want = 0;
-  if (sizeof(T) == 4 && groups > 16)
-    want = std::min((groups + xcd - 1) / xcd * xcd, cus / 2 / xcd * xcd);
   if (opt && opt->panel_cus > 0) want = std::min(opt->panel_cus, cus) / xcd * xcd;
   if (opt && opt->panel_cus < 0) want = 0;
   if (want >= cus) want = 0;                // no CUs left for the updates: share the device

@@ -104,8 +104,7 @@ typedef struct dlu_options {
   long long spin_limit_ticks; /* bound on any in-kernel wait, 100 MHz ticks (default 2e8 = 2 s) */
   int profile;     /* nonzero: time every panel / solve / update launch with HIP events */
   int panel_cus;   /* CUs reserved for panel kernels while updates run on the rest: > 0 = that
-                      many, < 0 = none, 0 = automatic (none for FP64; for FP32 above 4096 rows,
-                      one CU per panel workgroup, at most half the device) */
+                      many (whole XCDs), <= 0 = none (the default) */
   int rows_per_lane; /* panel rows held per lane: 0 = automatic, else 1 to 4 (the exchange-and-move
                         panel rounds 3 up to 4) */
   int merge_rows;    /* pairs of panels share one trailing GEMM (K = 2 nb) while at least this

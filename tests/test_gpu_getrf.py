@@ -337,14 +337,15 @@ def test_schedules_match_oracle(sched, case):
 
 
 def test_stream_pairs_of_other_partitions_are_replaced():
-    """FP32 factorizations above 4096 rows reserve panel CUs sized by their row count, so four of
-    different sizes fill a workspace's four stream pairs with four partitions.  An FP64 call on
-    the same thread asks for no partition: it must get an unpartitioned pair (a replaced slot),
-    not a partitioned one whose panel stream is too small for its 64 panel workgroups."""
+    """Four FP32 factorizations with four different panel CU reservations fill a workspace's four
+    stream pairs with four partitions.  An FP64 call on the same thread asks for no partition: it
+    must get an unpartitioned pair (a replaced slot), not a partitioned one whose panel stream is
+    too small for its panel workgroups."""
     torch = _torch()
-    for n in (5000, 10000, 16384, 20000):
+    for n, cus in ((5000, 32), (10000, 64), (16384, 96), (20000, 128)):
         t = torch.randn((n, n), dtype=torch.float32, device="cuda")
-        st, ipiv, jpiv, info, rep, _ = dlu.sgetrf_device(n, n, t.data_ptr(), n)
+        st, ipiv, jpiv, info, rep, _ = dlu.sgetrf_device(n, n, t.data_ptr(), n,
+                                                         options={"panel_cus": cus})
         torch.cuda.synchronize()
         assert st == dlu.Status.SUCCESS and info == n, (n, dlu.status_string(st))
         del t
