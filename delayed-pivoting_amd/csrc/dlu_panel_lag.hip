@@ -312,7 +312,9 @@ struct Lag {
   }
 };
 
-template <typename T, int NB, int RPL, int GQ>
+// INSTR: the launch may track margins or time its phases (P.margins, P.phase_ticks); the plain
+// variant compiles neither, which keeps the leader's step short.
+template <typename T, int NB, int RPL, int GQ, bool INSTR>
 __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P) {
   typedef typename Lag<T, NB, RPL>::row16_t row16_t;
   constexpr int WPV = Words<T>::n;
@@ -333,6 +335,7 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
   const int width = P.pe - P.pb;
   const int col0 = wave * 16;
   const int m = P.m;
+  const bool margins = INSTR && P.margins != nullptr;
 
   __shared__ LagShared<T, NB, RPL> S;
 
@@ -451,7 +454,7 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
       pos[r] = base + 64 * r + lane;
       if (pos[r] < m) elig |= 1u << r;
     }
-    const bool timed = P.phase_ticks != nullptr && g == 0 && lane == 0;
+    const bool timed = INSTR && P.phase_ticks != nullptr && g == 0 && lane == 0;
     if (timed) {
       for (int i = 0; i < 8; ++i) S.ph[i] = 0;
       S.ph[8] = wall_clock64();
@@ -582,7 +585,7 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
         put_halves(c1, h + H1);
         put_halves(c2, h + H2);
         put_halves(c3, h + HL);
-        if (P.margins != nullptr) {  // second-largest magnitude of this workgroup's rows
+        if (margins) {  // second-largest magnitude of this workgroup's rows
           uint64_t m2 = 0;
 #pragma unroll
           for (int r = 0; r < RPL; ++r) m2 = max(m2, pos[r] != cpos ? key[r] : 0ull);
@@ -596,7 +599,7 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
           const uint64_t wj = pack_word<NHALF>(h, j, sq);
           mine = (lane == j) ? wj : mine;
         }
-        if (lane < (P.margins != nullptr ? NHW : NCORE))
+        if (lane < (margins ? NHW : NCORE))
           st_word(hdr_slot(P.slots, t & 1, g) + lane, mine);
       }
       DLU_PHASE(2)
@@ -631,7 +634,7 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
         const int o0 = ((t & 1) * kLagMaxGroups + lane) * kHdrWords * 8;
         const int o1 = ((t & 1) * kLagMaxGroups + lane + 64) * kHdrWords * 8;
         const unsigned tg = (P.seq_base + static_cast<unsigned>(t) + 1u) & 0xFFFFu;
-        const int nw = P.margins != nullptr ? NHW : NCORE;
+        const int nw = margins ? NHW : NCORE;
         uint64_t hw0[NHW], hw1[NHW];
 #pragma unroll
         for (int j = 0; j < NHW; ++j) hw0[j] = hw1[j] = 0ull;
@@ -714,7 +717,7 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
             u1 = get_halves<T>(x + H1);
             w2 = get_halves<T>(x + H2);
             lw = get_halves<T>(x + HL);
-            if (g == 0 && P.margins != nullptr) {  // margin: the largest candidate elsewhere
+            if (g == 0 && margins) {  // margin: the largest candidate elsewhere
               const uint64_t o0k = (q0 && lane != gwin) ? k0 : 0ull;
               const uint64_t o1k = (q1 && lane + 64 != gwin) ? k1 : 0ull;
               const uint64_t o = wave_max_key(max(o0k, o1k));
@@ -794,14 +797,14 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
     if (g == 0) {
       for (int s = threadIdx.x; s < steps; s += blockDim.x) {
         if (!(P.positioned && s == 0)) P.ipiv[P.pb + s] = S.hist[s];
-        if (P.margins)
+        if (margins)
           P.margins[P.pb + s] =
               (P.positioned && s == 0) ? -1.0 : (S.g1[s] - fmax(S.g2[s], 0.0)) / S.g1[s];
       }
       if (wave == 0) {  // tie statistics of the column steps (when margins are tracked)
         double lo = 2.0;
         int near = 0, exact = 0;
-        for (int s0 = 0; P.margins != nullptr && s0 < steps; s0 += 64) {
+        for (int s0 = 0; margins && s0 < steps; s0 += 64) {
           const int s = s0 + lane;
           const bool valid = s < steps && !(P.positioned && s == 0);
           const double mg = valid ? (S.g1[s] - fmax(S.g2[s], 0.0)) / S.g1[s] : 2.0;
@@ -831,14 +834,14 @@ leave:
   }
 }
 
-template <typename T, int NB, int GQ>
+template <typename T, int NB, int GQ, bool INSTR>
 hipError_t launch_lag_gq(const PanelParams<T> &p, int rows_per_lane, hipStream_t s) {
   const dim3 grid(p.groups), block(NB * 4 + 64);
   switch (rows_per_lane) {
-    case 1: hipLaunchKernelGGL((panel_lag_kernel<T, NB, 1, GQ>), grid, block, 0, s, p); break;
-    case 2: hipLaunchKernelGGL((panel_lag_kernel<T, NB, 2, GQ>), grid, block, 0, s, p); break;
-    case 3: hipLaunchKernelGGL((panel_lag_kernel<T, NB, 3, GQ>), grid, block, 0, s, p); break;
-    case 4: hipLaunchKernelGGL((panel_lag_kernel<T, NB, 4, GQ>), grid, block, 0, s, p); break;
+    case 1: hipLaunchKernelGGL((panel_lag_kernel<T, NB, 1, GQ, INSTR>), grid, block, 0, s, p); break;
+    case 2: hipLaunchKernelGGL((panel_lag_kernel<T, NB, 2, GQ, INSTR>), grid, block, 0, s, p); break;
+    case 3: hipLaunchKernelGGL((panel_lag_kernel<T, NB, 3, GQ, INSTR>), grid, block, 0, s, p); break;
+    case 4: hipLaunchKernelGGL((panel_lag_kernel<T, NB, 4, GQ, INSTR>), grid, block, 0, s, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -847,8 +850,12 @@ hipError_t launch_lag_gq(const PanelParams<T> &p, int rows_per_lane, hipStream_t
 // The leader reads one header per lane up to 64 workgroups, two beyond.
 template <typename T, int NB>
 hipError_t launch_lag_nb(const PanelParams<T> &p, int rows_per_lane, hipStream_t s) {
-  return p.groups <= 64 ? launch_lag_gq<T, NB, 1>(p, rows_per_lane, s)
-                        : launch_lag_gq<T, NB, 2>(p, rows_per_lane, s);
+  const bool instr = p.margins != nullptr || p.phase_ticks != nullptr;
+  if (p.groups <= 64)
+    return instr ? launch_lag_gq<T, NB, 1, true>(p, rows_per_lane, s)
+                 : launch_lag_gq<T, NB, 1, false>(p, rows_per_lane, s);
+  return instr ? launch_lag_gq<T, NB, 2, true>(p, rows_per_lane, s)
+               : launch_lag_gq<T, NB, 2, false>(p, rows_per_lane, s);
 }
 
 }  // namespace
