@@ -513,24 +513,35 @@ struct Run {
     return single ? hipSuccess : hipEventRecord(w.ev_pan[p.done_seq % kDoneSlots], st);
   }
 
-  // A panel's control block: spin on the sequence number the panel kernel stores into its host
-  // slot as its last act.  A launch whose end event completes without that store has failed.
+  // A panel's outcome: spin on the word the panel kernel stores into its host slot as its last
+  // act (its sequence number in the top bits).  A launch whose end event completes without that
+  // store has failed.
   hipError_t wait_panel(const Launch &L, Control &c) {
     if (single || w.h_done == nullptr) {  // no pinned copy: read the device's
       hipError_t e = hipMemcpyAsync(&c, w.d_ctl, sizeof(Control), hipMemcpyDeviceToHost, s);
       return e == hipSuccess ? hipStreamSynchronize(s) : e;
     }
     PanelDone &d = w.h_done[L.seq % kDoneSlots];
+    const unsigned long long want = L.seq & kDoneSeqMask;
+    auto take = [&](unsigned long long word) {
+      c = Control{};
+      c.stop = static_cast<int>(static_cast<unsigned>(word));
+      c.zero_column = static_cast<int>((word >> 32) & 1u);
+      c.deferred = static_cast<int>((word >> 33) & 1u);
+      c.error = static_cast<int>((word >> 34) & 3u);
+    };
     for (unsigned it = 1;; ++it) {
-      if (__atomic_load_n(&d.seq, __ATOMIC_ACQUIRE) == L.seq) {
-        c = d.c;
+      unsigned long long word = __atomic_load_n(&d.word, __ATOMIC_ACQUIRE);
+      if ((word >> 36) == want) {
+        take(word);
         return hipSuccess;
       }
       if ((it & 255) == 0) {
         const hipError_t q = hipEventQuery(w.ev_pan[L.seq % kDoneSlots]);
         if (q == hipSuccess) {
-          if (__atomic_load_n(&d.seq, __ATOMIC_ACQUIRE) == L.seq) {
-            c = d.c;
+          word = __atomic_load_n(&d.word, __ATOMIC_ACQUIRE);
+          if ((word >> 36) == want) {
+            take(word);
             return hipSuccess;
           }
           return hipErrorLaunchFailure;

@@ -47,22 +47,28 @@ __device__ __forceinline__ bool skipped(const Pred &p) {
   return p.go != nullptr && *p.go != p.expect;
 }
 
-// A panel's control block as the host sees it: workgroup 0 of the panel kernel copies the
-// control block here (pinned, coherent host memory) as its last act, then stores `seq` with
-// system-scope release; the host spins on `seq` instead of waiting for a copy and an event.
+// What the host needs of a panel, as the panel publishes it into pinned, coherent host memory:
+// one 64-bit word written by one system-scope store as the panel's last act (no fence: the
+// fields travel together), which the host spins on.  Bits [0, 32) the stop, 32 zero column,
+// 33 deferred, [34, 36) the error code (capped at 3), [36, 64) the low 28 bits of the launch's
+// sequence number.  The rest of the control block (tie statistics, fallback count) is read once,
+// at the end of the factorization.
 struct PanelDone {
-  Control c;
-  unsigned long long seq;
+  unsigned long long word;
 };
+constexpr unsigned long long kDoneSeqMask = (1ull << 28) - 1;
 
-// The panel's last word to the host: the control block, copied into pinned host memory, then
-// the sequence number with system-scope release (the host spins on it).
 __device__ __forceinline__ void publish_done(const Control *ctl, PanelDone *done,
                                              unsigned long long seq) {
   if (done == nullptr) return;
-  done->c = *ctl;
-  __threadfence_system();
-  __hip_atomic_store(&done->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  const int err = ctl->error;
+  const unsigned long long w =
+      static_cast<unsigned long long>(static_cast<unsigned>(ctl->stop)) |
+      (static_cast<unsigned long long>(ctl->zero_column != 0) << 32) |
+      (static_cast<unsigned long long>(ctl->deferred != 0) << 33) |
+      (static_cast<unsigned long long>(err < 0 ? 3 : (err > 3 ? 3 : err)) << 34) |
+      ((seq & kDoneSeqMask) << 36);
+  __hip_atomic_store(&done->word, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Exchange buffer geometry of the panel kernel (in 64-bit words).  Every word carries the step
