@@ -842,7 +842,9 @@ struct Run {
   // Those launches are predicated on panel j's token (Control::go) and do nothing when the guess
   // fails, so the panel chain runs without a host round trip or a cross-stream wait between
   // panels.  `ev_far` (everything `us` holds) orders each near work after the far work before it;
-  // `ev_near` orders the far work of a step after its near work.
+  // `ev_near` orders the far work of a pair's second step after its near work (the other far
+  // work reads nothing the near work writes, and waits for the panel alone), so between two
+  // panels `ps` holds one wait and one record besides the kernels.
   hipError_t factor(int *info, bool lookahead) {
     hipError_t e;
     if (single) lookahead = false;
@@ -951,11 +953,14 @@ struct Run {
           if ((e = hipStreamWaitEvent(ps, w.ev_pan[cur.seq % kDoneSlots], 0)) != hipSuccess) return e;
           if ((e = hipStreamWaitEvent(ps, w.ev_far, 0)) != hipSuccess) return e;
           if ((e = near(st, cur, ps)) != hipSuccess) return e;
-          if ((e = hipEventRecord(w.ev_near, ps)) != hipSuccess) return e;
+          if (st == Step::Pair2 && (e = hipEventRecord(w.ev_near, ps)) != hipSuccess) return e;
           if ((e = panel(pe, pe2, 0, ps)) != hipSuccess) return e;
           nxt = launched;
         }
-        if ((e = hipStreamWaitEvent(us, w.ev_near, 0)) != hipSuccess) return e;
+        // Only a pair's second step has far work that reads what its near work wrote (P0's L
+        // columns after P1's interchange); the others need the panel alone (waited for above).
+        if (st == Step::Pair2 && (e = hipStreamWaitEvent(us, w.ev_near, 0)) != hipSuccess)
+          return e;
         if ((e = far(st, cur)) != hipSuccess) return e;
         if ((e = hipEventRecord(w.ev_far, us)) != hipSuccess) return e;
         cur = nxt;
@@ -967,7 +972,7 @@ struct Run {
           pred = Pred{&w.d_ctl->go, cur.seq};
           e = hipStreamWaitEvent(ps, w.ev_far, 0);
           if (e == hipSuccess) e = near(sn, cur, ps);
-          if (e == hipSuccess) e = hipEventRecord(w.ev_near, ps);
+          if (e == hipSuccess && sn == Step::Pair2) e = hipEventRecord(w.ev_near, ps);
           if (e == hipSuccess) e = panel(cur.pe, std::min(cur.pe + nb, steps), 0, ps);
           pred = Pred{nullptr, 0};
           if (e != hipSuccess) return e;
