@@ -253,11 +253,12 @@ __global__ __launch_bounds__(256, OCC) void gemm_sub_kernel(T *__restrict__ C, i
 // are formed by the wave itself in LDS.  Rows at and beyond w are zero-padded and never
 // written back.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int NB, bool LIST, bool FULL>
+template <typename T, int NB, bool LIST, bool FULL, bool TWO = false>
 __global__ __launch_bounds__(64) void swap_trsm_kernel(T *__restrict__ a, int lda, int pb, int w,
                                                       int c0, int c1,
                                                       const int *__restrict__ swap_list, int ls,
-                                                      Pred pred) {
+                                                      Pred pred,
+                                                      const int *__restrict__ list2) {
   if (skipped(pred)) return;
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
@@ -406,6 +407,46 @@ __global__ __launch_bounds__(64) void swap_trsm_kernel(T *__restrict__ a, int ld
       const int i = 64 * j + l;
       if (i < w && c < ncol) S[static_cast<size_t>(pb + i) + static_cast<size_t>(c) * lda] = X[i][c];
     }
+  if (!TWO) return;
+
+  // 6. TWO (the first panel of a pair, full): the second panel's movement (list2, rows from
+  // pb + NB on) on the same strip, which reads rows this wave's displaced-row stores may just
+  // have written (its own stores: a workgroup-scope fence orders them).
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __syncthreads();
+  const int pb2 = pb + NB;
+  const int cnt2 = list2[4 * ls];
+  const int *src2 = list2 + 2 * ls;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int idx = NB + 64 * j + l;
+    const bool ok = idx < cnt2;
+    edst[j] = ok ? list2[idx] : -1;
+    const int es = ok ? src2[idx] : pb2;
+#pragma unroll
+    for (int c = 0; c < 16; ++c)
+      e[j][c] = S[static_cast<size_t>(es) + static_cast<size_t>(c < cmax ? c : cmax) * lda];
+  }
+#pragma unroll
+  for (int c = 0; c < 16; ++c)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int row = src2[64 * j + l];
+      X[64 * j + l][c] = S[static_cast<size_t>(row) + static_cast<size_t>(c < cmax ? c : cmax) * lda];
+    }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+    if (edst[j] >= 0)
+#pragma unroll
+      for (int c = 0; c < 16; ++c)
+        if (c < ncol) S[static_cast<size_t>(edst[j]) + static_cast<size_t>(c) * lda] = e[j][c];
+#pragma unroll
+  for (int c = 0; c < 16; ++c)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+      if (c < ncol)
+        S[static_cast<size_t>(pb2 + 64 * j + l) + static_cast<size_t>(c) * lda] = X[64 * j + l][c];
 }
 
 template <typename T, int BM, int BN, int BK, int OCC>
@@ -490,16 +531,16 @@ hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const 
   case NBV:                                                                                      \
     if (swap_list && w == NBV)                                                                   \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, true, true>), grid, block, 0, s, a, lda, pb,  \
-                         w, c0, c1, swap_list, nb, pred);                                        \
+                         w, c0, c1, swap_list, nb, pred, nullptr);                                        \
     else if (swap_list)                                                                          \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, true, false>), grid, block, 0, s, a, lda, pb, \
-                         w, c0, c1, swap_list, nb, pred);                                        \
+                         w, c0, c1, swap_list, nb, pred, nullptr);                                        \
     else if (w == NBV)                                                                           \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, false, true>), grid, block, 0, s, a, lda, pb, \
-                         w, c0, c1, swap_list, nb, pred);                                        \
+                         w, c0, c1, swap_list, nb, pred, nullptr);                                        \
     else                                                                                         \
       hipLaunchKernelGGL((swap_trsm_kernel<T, NBV, false, false>), grid, block, 0, s, a, lda,    \
-                         pb, w, c0, c1, swap_list, nb, pred);                                    \
+                         pb, w, c0, c1, swap_list, nb, pred, nullptr);                                    \
     break;
   if (nb != 64 && nb != 128 && nb != 256) return hipErrorInvalidValue;
   switch (tile) {
@@ -513,11 +554,36 @@ hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const 
 }
 
 
+template <typename T>
+hipError_t launch_swap_trsm_pair(T *a, int lda, int b0, int c0, int c1, const int *l0,
+                                 const int *l1, int nb, hipStream_t s, Pred pred) {
+  if (c1 <= c0) return hipSuccess;
+  const dim3 grid((c1 - c0 + 15) / 16), block(64);
+  switch (nb) {
+    case 64:
+      hipLaunchKernelGGL((swap_trsm_kernel<T, 64, true, true, true>), grid, block, 0, s, a, lda,
+                         b0, 64, c0, c1, l0, nb, pred, l1);
+      break;
+    case 128:
+      hipLaunchKernelGGL((swap_trsm_kernel<T, 128, true, true, true>), grid, block, 0, s, a, lda,
+                         b0, 128, c0, c1, l0, nb, pred, l1);
+      break;
+    case 256:
+      hipLaunchKernelGGL((swap_trsm_kernel<T, 256, true, true, true>), grid, block, 0, s, a, lda,
+                         b0, 256, c0, c1, l0, nb, pred, l1);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 #define DLU_BLAS3_INSTANTIATE(T)                                                                 \
   template hipError_t launch_gemm_sub<T>(T *, int, const T *, int, const T *, int, int, int, int, \
                                          hipStream_t, int, Pred, AmaxOut);                       \
   template hipError_t launch_swap_trsm<T>(T *, int, int, int, int, int, const int *, int,        \
-                                          hipStream_t, Pred);
+                                          hipStream_t, Pred);                                    \
+  template hipError_t launch_swap_trsm_pair<T>(T *, int, int, int, int, const int *, const int *, \
+                                               int, hipStream_t, Pred);
 DLU_BLAS3_INSTANTIATE(float)
 DLU_BLAS3_INSTANTIATE(double)
 #undef DLU_BLAS3_INSTANTIATE

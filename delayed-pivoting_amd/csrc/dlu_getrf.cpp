@@ -615,8 +615,10 @@ struct Run {
   // Columns [c0, c1) up to (but without) the K = 2 nb GEMM.
   hipError_t pair_prepare(int b0, const int *l0, const int *l1, int c0, int c1, hipStream_t st) {
     const int b1 = b0 + nb;
-    hipError_t e = solve(b0, nb, c0, c1, l0, st);
-    if (e == hipSuccess) e = launch_row_interchange(a, lda, c0, c1, l1, nb, st, pred);
+    // P0's interchange and U block row, then P1's interchange, in one pass over the strips
+    hipError_t e = profile ? mark(t_solve, st, 0.0) : hipSuccess;
+    if (e == hipSuccess) e = launch_swap_trsm_pair(a, lda, b0, c0, c1, l0, l1, nb, st, pred);
+    if (e == hipSuccess && profile) e = close(t_solve, st);
     if (e == hipSuccess) e = gemm(b1, c0, b0, b0, nb, c1 - c0, nb, st);
     if (e == hipSuccess) e = solve(b1, nb, c0, c1, nullptr, st);
     return e;

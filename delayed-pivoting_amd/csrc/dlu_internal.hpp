@@ -207,6 +207,12 @@ template <typename T>
 hipError_t launch_swap_trsm(T *a, int lda, int pb, int w, int c0, int c1, const int *swap_list,
                             int nb, hipStream_t s, Pred pred = Pred{nullptr, 0});
 
+// The first panel of a full pair (rows from b0, list l0) as launch_swap_trsm does it, then the
+// second panel's movement (rows from b0 + nb, list l1) on the same columns, in one launch.
+template <typename T>
+hipError_t launch_swap_trsm_pair(T *a, int lda, int b0, int c0, int c1, const int *l0,
+                                 const int *l1, int nb, hipStream_t s, Pred pred);
+
 // C(M x N) -= A(M x K) * B(K x N), column-major with leading dimensions ldc, lda, ldb.
 // gemm_uses_large_tiles names the launches the profiled report counts as large (at least 512
 // blocks of 128 x 128 of C: the far updates while many rows remain).
