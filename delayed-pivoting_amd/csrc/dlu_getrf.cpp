@@ -188,10 +188,17 @@ inline void panel_shape_for(int rows, int nb, int forced_rpl, int max_groups, in
 inline bool lag_shape_for(int rows, int nb, size_t elem, int forced_rpl, int max_groups,
                           int &groups, int &rpl) {
   if (nb != 64 && nb != 128) return false;
-  // FP64 holds two rows per lane in registers; above 10000 rows a third (in LDS) takes a third
-  // fewer CUs from the trailing GEMM that runs beside the panel while the update is the longer
-  // chain (n = 16384: 100.7 -> 99.9 ms).
-  rpl = forced_rpl > 0 ? forced_rpl : (elem == 8 ? (rows > 10000 ? 3 : 2) : 4);
+  // Rows per lane: fewer make a shorter step, more leave more CUs to the trailing GEMM that runs
+  // beside the panel while the update is the longer chain.  FP64: 2 in registers, 3 (one in
+  // LDS) above 10000 rows (n = 16384: 100.7 -> 99.9 ms), 1 up to 4096 rows (at most 64
+  // workgroups: n = 2048 5.94 -> 5.45 ms, n = 4096 12.37 -> 11.66 ms).  FP32: 4, and 2 up to
+  // 4096 rows (n = 2048: 5.89 -> 5.18 ms; at n = 16384 4 is best: 68.1 against 71.3 ms).
+  if (forced_rpl > 0)
+    rpl = forced_rpl;
+  else if (elem == 8)
+    rpl = rows > 10000 ? 3 : (rows > 4096 ? 2 : 1);
+  else
+    rpl = rows > 4096 ? 4 : 2;
   groups = (rows + 64 * rpl - 1) / (64 * rpl);
   return groups <= std::min(kLagMaxGroups, max_groups);
 }
