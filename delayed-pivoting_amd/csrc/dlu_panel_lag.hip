@@ -466,15 +466,16 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
     S.ph[8] = tn;                                 \
   }
     const int G = P.groups;
-    // The winner of the last reduced step: pivot, its W1 (column t), W2 and Lp, position.
-    T piv = T(0), u1 = T(0), w2 = T(0), lw = T(0);
+    // The winner of the last reduced step: 1 / pivot, its W1 (column t), W2 and Lp, position.
+    T rcp = T(0), u1 = T(0), w2 = T(0), lw = T(0);
     int gwin = 0, p = 0;
     bool zero_now = false;
+    bool owner_failed = false;  // an owner's failure as read while the last header travelled
     for (int t = 0;; ++t) {
       const int k = P.pb + t;
       tvec l, n0;  // multipliers of step t-1; column t through step t-1
       if (t > 0) {
-        if (__hip_atomic_load(&S.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) failed = true;
+        if (owner_failed) failed = true;
         if (lane == 0) {
           S.win[(t - 1) & 1][0] = gwin;
           S.win[(t - 1) & 1][1] = p;
@@ -485,10 +486,9 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
           __syncthreads();
           break;
         }
-        const T recip = T(1) / piv;
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
-          l[r] = W0[r] * recip;
+          l[r] = W0[r] * rcp;
           n0[r] = fma(-l[r], u1, W1[r]);
         }
       } else {
@@ -557,9 +557,14 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
         }
       }
       const bool has = lk != 0;
-      // Step t-1's update of column t+1 needs U(t-2, t+1) for the lagged W2 (the owners' copy).
+      // Step t-1's update of column t+1 needs U(t-2, t+1) for the lagged W2 (the owners' copy);
+      // column t+2 through step t-2 is the owners' hand-over.  Both LDS reads go out together.
       const bool lag = t >= 2 && t + 1 < width;
       const T up = lag ? S.u[(t - 2) & 1][t + 1] : T(0);
+      tvec n2;
+#pragma unroll
+      for (int r = 0; r < RPL; ++r)
+        n2[r] = (t > 0 && t + 2 < width) ? S.h[(t - 1) & 1][64 * r + lane] : T(0);
       const T u2 = lag ? fma(-lw, up, w2) : w2;  // U(t-1, t+1)
 
       // The header: the candidate's window values, computed for it alone first.
@@ -571,7 +576,7 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
           const T cw2 = lane_value(W2[cr], cl), clp = lane_value(Lp[cr], cl);
           c3 = lane_value(l[cr], cl);
           c1 = fma(-c3, u2, lag ? fma(-clp, up, cw2) : cw2);
-          c2 = t + 2 < width ? S.h[(t - 1) & 1][64 * cr + cl] : T(0);
+          c2 = lane_value(n2[cr], cl);
         } else {
           c1 = lane_value(W1[cr], cl);
           c2 = lane_value(W2[cr], cl);
@@ -610,17 +615,18 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
         for (int r = 0; r < RPL; ++r) {
           const T w2c = lag ? fma(-Lp[r], up, W2[r]) : W2[r];
           const T n1 = fma(-l[r], u2, w2c);
-          const T n2 = t + 2 < width ? S.h[(t - 1) & 1][64 * r + lane] : T(0);
           S.l[(t - 1) & 1][64 * r + lane] = l[r];
           W0[r] = n0[r];
           W1[r] = n1;
-          W2[r] = n2;
+          W2[r] = n2[r];
           Lp[r] = l[r];
         }
         lds_signal(&S.flag_l, t - 1);
       }
       S.cand[t & 1] = has ? 64 * cr + cl : -1;
       lds_signal(&S.flag_c, t);  // the candidate, and the header is out
+      owner_failed =
+          __hip_atomic_load(&S.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
       DLU_PHASE(3)
 
       // Every workgroup's header: lane i reads workgroups i and i + 64.  Each poll loads the
@@ -667,6 +673,9 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
           const bool use1 = k1 > k0 || (k1 == k0 && p1 < p0);
           const uint64_t bk = use1 ? k1 : k0;
           const int bp = use1 ? p1 : p0;
+          // 1 / pivot of this lane's better header, formed on every lane while the reduction
+          // runs; the winner's lane holds the step's value.
+          const T lrcp = T(1) / (use1 ? get_halves<T>(a1) : get_halves<T>(a0));
           const uint64_t gmax = wave_max_key_fast(bk);
           if (gmax == 0) {  // zero (or NaN-only) column: stop the panel at k
             zero_now = true;
@@ -713,7 +722,7 @@ __global__ __launch_bounds__(NB * 4 + 64) void panel_lag_kernel(PanelParams<T> P
             }
             unsigned x[NHALF];
             unpack_words<NHALF>(ww, x);
-            piv = get_halves<T>(x);
+            rcp = lane_value(lrcp, wlane);
             u1 = get_halves<T>(x + H1);
             w2 = get_halves<T>(x + H2);
             lw = get_halves<T>(x + HL);
