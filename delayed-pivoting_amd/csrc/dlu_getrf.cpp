@@ -99,6 +99,7 @@ struct Workspace {
   hipEvent_t ev_solved = nullptr, ev_tail = nullptr;  // far U block row done / tail GEMM done
   hipEvent_t ev_pan[4] = {};                          // panel launch seq ends: ev_pan[seq % 4]
   hipEvent_t ev_near = nullptr, ev_far = nullptr;     // panel-stream near work / update-stream work
+  hipEvent_t ev_pair = nullptr;  // a pair's second interchange on the first panel's columns
   bool ready = false;
 };
 
@@ -272,7 +273,7 @@ hipError_t prepare(Workspace &w, int device, int steps) {
     if ((e = hipMalloc(&w.d_phase, 8 * sizeof(unsigned long long))) != hipSuccess) return e;
     for (hipEvent_t *ev : {&w.ev_fork, &w.ev_next, &w.ev_join, &w.ev_solved, &w.ev_tail,
                            &w.ev_pan[0], &w.ev_pan[1], &w.ev_pan[2], &w.ev_pan[3], &w.ev_near,
-                           &w.ev_far})
+                           &w.ev_far, &w.ev_pair})
       if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
     w.ready = true;
@@ -791,12 +792,17 @@ struct Run {
     const int pe2 = std::min(L.pe + nb, steps);
     if (st == Step::Pair2) {
       hipError_t e = launch_row_interchange(a, lda, pair.b0, L.pb, L.list, nb, q, pred);
+      // The far columns' interchanges and solve need P0's columns after this interchange only.
+      if (e == hipSuccess && q != us) e = hipEventRecord(w.ev_pair, q);
       return e == hipSuccess ? pair_update(pair.b0, pair.l0, L.list, L.pe, pe2, q) : e;
     }
     return update(L.pb, nb, L.pe, pe2, L.list, q);
   }
-  // The rest of the step, on `us`, once its near work is done.
-  hipError_t far(Step st, const Launch &L) {
+  // The rest of the step, on `us`.  With the near work on the panel stream (near_on_ps), a
+  // pair's second step waits there for P1's interchange on P0's columns (`ev_pair`) before its
+  // interchanges and solve, and for the whole near work (`ev_near`) before its GEMM, so that the
+  // next panel is resident before the GEMM fills the GPU.
+  hipError_t far(Step st, const Launch &L, bool near_on_ps = false) {
     const int pb = L.pb, pe = L.pe, pe2 = std::min(pe + nb, steps);
     hipError_t e;
     if (st == Step::Pair2) {
@@ -808,7 +814,12 @@ struct Run {
       if ((e = wait_tail(us)) != hipSuccess) return e;
       const int half = std::max(nb, (n - pe2) / 2 / nb * nb);
       const int cm = std::min(n, pe2 + half);
-      if ((e = pair_update(b0, pair.l0, L.list, pe2, cm, us)) != hipSuccess) return e;
+      if (near_on_ps && (e = hipStreamWaitEvent(us, w.ev_pair, 0)) != hipSuccess) return e;
+      if (cm > pe2 && (e = pair_prepare(b0, pair.l0, L.list, pe2, cm, us)) != hipSuccess) return e;
+      if (near_on_ps && (e = hipStreamWaitEvent(us, w.ev_near, 0)) != hipSuccess) return e;
+      if (cm > pe2 && (e = gemm(b0 + 2 * nb, pe2, b0, b0, m - b0 - 2 * nb, cm - pe2, 2 * nb, us)) !=
+                          hipSuccess)
+        return e;
       rest = Rest{true, b0, cm, pair.l0, L.list};
       if (defer) {  // P0's columns took P1's movement in the near work
         blocks.push_back(Block{b0, pb, L.event + 1});
@@ -851,9 +862,11 @@ struct Run {
   // Those launches are predicated on panel j's token (Control::go) and do nothing when the guess
   // fails, so the panel chain runs without a host round trip or a cross-stream wait between
   // panels.  `ev_far` (everything `us` holds) orders each near work after the far work before it;
-  // `ev_near` orders the far work of a pair's second step after its near work (the other far
-  // work reads nothing the near work writes, and waits for the panel alone), so between two
-  // panels `ps` holds one wait and one record besides the kernels.
+  // a pair's second step records `ev_pair` after its first near kernel (P1's interchange on P0's
+  // columns) and `ev_near` after the near work: its far interchanges and solve wait for the
+  // first, its far GEMM for the second (the other far work reads nothing the near work writes,
+  // and waits for the panel alone).  Between two panels `ps` holds one wait and one record
+  // besides the kernels (three records after a pair's second panel).
   hipError_t factor(int *info, bool lookahead) {
     hipError_t e;
     if (single) lookahead = false;
@@ -967,10 +980,9 @@ struct Run {
           nxt = launched;
         }
         // Only a pair's second step has far work that reads what its near work wrote (P0's L
-        // columns after P1's interchange); the others need the panel alone (waited for above).
-        if (st == Step::Pair2 && (e = hipStreamWaitEvent(us, w.ev_near, 0)) != hipSuccess)
-          return e;
-        if ((e = far(st, cur)) != hipSuccess) return e;
+        // columns after P1's interchange, waited for inside); the others need the panel alone
+        // (waited for above).
+        if ((e = far(st, cur, true)) != hipSuccess) return e;
         if ((e = hipEventRecord(w.ev_far, us)) != hipSuccess) return e;
         cur = nxt;
         nxt = Launch{};
@@ -1025,7 +1037,7 @@ void release_workspace(Workspace &w) {
     if (w.streams[i].update) hipStreamDestroy(w.streams[i].update);
   }
   for (hipEvent_t ev : {w.ev_fork, w.ev_next, w.ev_join, w.ev_solved, w.ev_tail, w.ev_pan[0],
-                        w.ev_pan[1], w.ev_pan[2], w.ev_pan[3], w.ev_near, w.ev_far})
+                        w.ev_pan[1], w.ev_pan[2], w.ev_pan[3], w.ev_near, w.ev_far, w.ev_pair})
     if (ev) hipEventDestroy(ev);
   for (void *p : {static_cast<void *>(w.d_ctl), static_cast<void *>(w.d_ipiv),
                   static_cast<void *>(w.d_jpiv), static_cast<void *>(w.d_margins),
