@@ -191,13 +191,15 @@ inline bool lag_shape_for(int rows, int nb, size_t elem, int forced_rpl, int max
   if (nb != 64 && nb != 128) return false;
   // Rows per lane: fewer make a shorter step, more leave more CUs to the trailing GEMM that runs
   // beside the panel while the update is the longer chain.  FP64: 2 in registers, 3 (one in
-  // LDS) above 10000 rows (n = 16384: 100.7 -> 99.9 ms), 1 up to 4096 rows (at most 64
-  // workgroups: n = 2048 5.94 -> 5.45 ms, n = 4096 12.37 -> 11.66 ms).  FP32: 4, and 2 up to
-  // 4096 rows (n = 2048: 5.89 -> 5.18 ms; at n = 16384 4 is best: 68.1 against 71.3 ms).
+  // LDS) above 10000 rows (n = 16384: 100.7 -> 99.9 ms), 4 (two in LDS) above 12288 rows, where
+  // the panel has most slack beside the GEMM (n = 16384: 95.7-96.0 -> 95.4 ms; 4 from 10000
+  // rows slows n = 12288), 1 up to 4096 rows (at most 64 workgroups: n = 2048 5.94 -> 5.45 ms,
+  // n = 4096 12.37 -> 11.66 ms).  FP32: 4, and 2 up to 4096 rows (n = 2048: 5.89 -> 5.18 ms; at
+  // n = 16384 4 is best: 68.1 against 71.3 ms).
   if (forced_rpl > 0)
     rpl = forced_rpl;
   else if (elem == 8)
-    rpl = rows > 10000 ? 3 : (rows > 4096 ? 2 : 1);
+    rpl = rows > 12288 ? 4 : (rows > 10000 ? 3 : (rows > 4096 ? 2 : 1));
   else
     rpl = rows > 4096 ? 4 : 2;
   groups = (rows + 64 * rpl - 1) / (64 * rpl);
