@@ -74,12 +74,10 @@ struct Workspace {
   unsigned long long *d_phase = nullptr;
   unsigned seq = 0;
   // Deferred left interchanges: every panel's swap list kept (archive), the composed
-  // permutation and its inverse, its copies per closed block, and the per-column plan (pinned
-  // host staging and its device copy).
+  // permutation's copies per closed block, and the per-column plan (pinned host staging and its
+  // device copy).
   int *d_archive = nullptr;
   int archive_cap = 0;  // lists
-  int *d_perm = nullptr;
-  int perm_cap = 0;     // rows
   int *d_snaps = nullptr;
   size_t snaps_cap = 0;  // ints
   int *d_plan = nullptr;
@@ -217,13 +215,6 @@ hipError_t prepare_defer(Workspace &w, int m, int n, int lists) {
         hipSuccess)
       return e;
     w.archive_cap = lists;
-  }
-  if (m > w.perm_cap) {
-    hipFree(w.d_perm);
-    w.d_perm = nullptr;
-    w.perm_cap = 0;
-    if ((e = hipMalloc(&w.d_perm, 2ull * m * sizeof(int))) != hipSuccess) return e;
-    w.perm_cap = m;
   }
   const size_t snaps = static_cast<size_t>(lists) * m;
   if (snaps > w.snaps_cap) {
@@ -707,8 +698,7 @@ struct Run {
     hipError_t e = hipMemcpyAsync(w.d_plan, w.h_plan, (E + 1 + 2 * static_cast<size_t>(n)) * sizeof(int),
                                   hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-      e = launch_left_compose(m, w.d_archive, kListStride, nb, E, w.d_plan, w.d_perm,
-                              w.d_perm + m, w.d_snaps, st);
+      e = launch_left_compose(m, w.d_archive, kListStride, nb, E, w.d_plan, w.d_snaps, st);
     if (e == hipSuccess)
       e = launch_left_gather<T>(a, lda, m, ncols, w.d_plan + E + 1, w.d_plan + E + 1 + n,
                                 w.d_snaps, st);
@@ -1047,8 +1037,8 @@ void release_workspace(Workspace &w) {
                   static_cast<void *>(w.d_lists), static_cast<void *>(w.d_pv),
                   static_cast<void *>(w.d_po), static_cast<void *>(w.d_phase),
                   static_cast<void *>(w.d_gpv), static_cast<void *>(w.d_gpo),
-                  static_cast<void *>(w.d_archive), static_cast<void *>(w.d_perm),
-                  static_cast<void *>(w.d_snaps), static_cast<void *>(w.d_plan)})
+                  static_cast<void *>(w.d_archive), static_cast<void *>(w.d_snaps),
+                  static_cast<void *>(w.d_plan)})
     if (p) hipFree(p);
   if (w.h_plan) hipHostFree(w.h_plan);
   if (w.h_ctl) hipHostFree(w.h_ctl);

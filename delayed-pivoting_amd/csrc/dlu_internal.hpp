@@ -243,7 +243,7 @@ hipError_t launch_row_interchange(T *a, int lda, int c0, int c1, const int *swap
 // gather rows [col_r0[c], m) of each column c < ncols with col_slot[c] >= 0 by snaps[slot].
 constexpr int kDeferMaxRows = 16384;
 hipError_t launch_left_compose(int m, const int *lists, int stride, int nb, int events,
-                               const int *snap_slot, int *P, int *Pinv, int *snaps, hipStream_t s);
+                               const int *snap_slot, int *snaps, hipStream_t s);
 template <typename T>
 hipError_t launch_left_gather(T *a, int lda, int m, int ncols, const int *col_slot,
                               const int *col_r0, const int *snaps, hipStream_t s);

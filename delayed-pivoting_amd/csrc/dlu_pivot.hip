@@ -354,36 +354,54 @@ __global__ __launch_bounds__(256) void row_interchange_kernel(T *__restrict__ a,
 // e ends up with the content row P(i) holds now: P <- q_e o P, with q_e(dst) = src) and copies
 // P out wherever a block needs it.  Each list moves a set of rows onto itself, so the update of
 // P is two parallel phases through its inverse.  Rows below a block's last row are the only
-// ones the later movements touch.
+// ones the later movements touch.  P and its inverse live in LDS (at most kDeferMaxRows rows),
+// and the next movement's list is loaded while the current one is applied, so a movement costs
+// two LDS phases and two barriers.
 __global__ __launch_bounds__(1024) void left_compose_kernel(int m, const int *__restrict__ lists,
                                                            int stride, int nb, int events,
                                                            const int *__restrict__ snap_slot,
-                                                           int *P, int *Pinv, int *snaps) {
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                                                           int *__restrict__ snaps) {
+  __shared__ int P[kDeferMaxRows], Pinv[kDeferMaxRows];
+  const int t = threadIdx.x;
+  for (int i = t; i < m; i += blockDim.x) {
     P[i] = i;
     Pinv[i] = i;
   }
-  __threadfence_block();
+  // Entry k of a list is read by thread k (at most 2 nb <= 512 entries).
+  const bool reader = t < 2 * nb;
+  auto load = [&](int e, int &cnt, int &dst, int &src) {
+    const int *L = lists + static_cast<size_t>(e - 1) * stride;
+    cnt = L[4 * nb];
+    dst = reader ? L[t] : 0;
+    src = reader ? L[2 * nb + t] : 0;
+  };
+  int cnt = 0, dst = 0, src = 0;
+  if (events >= 1) load(events, cnt, dst, src);
   __syncthreads();
   for (int e = events; e >= 1; --e) {
+    int ncnt = 0, ndst = 0, nsrc = 0;
+    if (e > 1) load(e - 1, ncnt, ndst, nsrc);
     const int slot = snap_slot[e];
-    if (slot >= 0)
-      for (int i = threadIdx.x; i < m; i += blockDim.x) snaps[static_cast<size_t>(slot) * m + i] = P[i];
-    const int *L = lists + static_cast<size_t>(e - 1) * stride;
-    const int cnt = L[4 * nb];  // at most 2 nb <= 512 entries: one per thread
-    const bool mine = static_cast<int>(threadIdx.x) < cnt;
-    int at = 0, src = 0;
-    if (mine) {
-      at = Pinv[L[threadIdx.x]];
-      src = L[2 * nb + threadIdx.x];
+    if (slot >= 0) {
+      int *out = snaps + static_cast<size_t>(slot) * m;
+      if ((m & 3) == 0) {  // 16-byte stores (m rows of ints, slot-aligned to 16 bytes)
+        for (int i = t; i < m / 4; i += blockDim.x)
+          reinterpret_cast<int4 *>(out)[i] = reinterpret_cast<const int4 *>(P)[i];
+      } else {
+        for (int i = t; i < m; i += blockDim.x) out[i] = P[i];
+      }
     }
+    const bool mine = t < cnt;
+    const int at = mine ? Pinv[dst] : 0;
     __syncthreads();
     if (mine) {
       P[at] = src;
       Pinv[src] = at;
     }
-    __threadfence_block();
     __syncthreads();
+    cnt = ncnt;
+    dst = ndst;
+    src = nsrc;
   }
 }
 
@@ -488,10 +506,10 @@ hipError_t launch_fallback_apply(T *a, int lda, int m, int n, int k, int *ipiv, 
 }
 
 hipError_t launch_left_compose(int m, const int *lists, int stride, int nb, int events,
-                               const int *snap_slot, int *P, int *Pinv, int *snaps,
-                               hipStream_t s) {
+                               const int *snap_slot, int *snaps, hipStream_t s) {
+  if (m > kDeferMaxRows || 2 * nb > 1024) return hipErrorInvalidValue;  // the kernel's LDS / threads
   hipLaunchKernelGGL(left_compose_kernel, dim3(1), dim3(1024), 0, s, m, lists, stride, nb, events,
-                     snap_slot, P, Pinv, snaps);
+                     snap_slot, snaps);
   return hipGetLastError();
 }
 
