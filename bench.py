@@ -482,7 +482,7 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                 "traffic": traffic, "algorithmic_bytes_per_launch": round(alg_bytes),
-                "kernel": "gemm_sub_kernel<double,128,128,16,true,2> (A22 -= L21*U12, FP64 MFMA, "
+                "kernel": "gemm_sub_kernel<double,128,128,16,true,2,false> (A22 -= L21*U12, FP64 MFMA, "
                           "K = 128 single panels and K = 256 panel pairs)",
                 "launches": prep["large_update_launches"],
                 "mean_launch_ms": round(upd_launch_s * 1e3, 4),
